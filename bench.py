@@ -1,0 +1,152 @@
+"""Assembly benchmark: assembled CSR nnz/s of the forward Jacobian at 64^3 P2 on one GPU.
+
+A step is one full assembly of the block A = d2R/dp du of ``exp(m) grad u . grad p - f p``
+on the unit cube in 64^3 hexahedra, state Q2 / parameter Q1, 3x3x3 Gauss rule, Dirichlet
+u = 0 on x = 0 and x = 1 (folded elimination, diagonal 1): the element kernel (geometry,
+fields, pointwise blocks, contraction) into resident element matrices and the scatter into
+hypre-layout CSR values.  Inputs (mesh, fields, pattern, plan) are resident in HBM before
+the timed region; the pattern build is one-time set-up, as in the reference's warm-up
+(``benchmarks/bench_assembly.py:10-17``).
+
+Prints one JSON line.  ``--gpus`` above 1 is refused: this path is measured on one GPU.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+METRIC = ("assembled CSR nnz/s + % HBM roofline at 64³ P2 (FP64 vs FP32-MFMA); "
+          "p50/p99 latency")
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=64, help="hexahedra per direction")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-n", type=int, default=12, help="mesh of the CPU baseline sample")
+    return ap.parse_args()
+
+
+def cpu_baseline(n):
+    """The oracle (numpy restatement of the reference's host path) on an n^3 sample:
+    element kernel in 2048-element windows (``kernel.py:659-672``), ``np.bincount``
+    scatter (``pattern.py:788``), folded elimination; pattern build excluded."""
+    try:
+        from threadpoolctl import threadpool_limits
+    except ImportError:                                             # pragma: no cover
+        threadpool_limits = None
+    from oracle import fe_oracle as O
+
+    P = O.Problem(n)
+    ip, ix, slot, nnz = O.pattern_tables_sort(P.Vu.edofs, P.Vu.edofs, P.Vu.ndof, P.Vu.ndof,
+                                              True)
+    mask = O.ess_mask(P.Vu.ndof, P.ess)
+    kill = O.masked_slots(ip, ix, mask, mask)
+
+    def once():
+        E = np.empty((P.mesh.ne, 27, 27))
+        for a in range(0, P.mesh.ne, 2048):
+            el = np.arange(a, min(a + 2048, P.mesh.ne))
+            E[a:a + el.size] = P.block(2, 0, el)
+        d = O.scatter(slot, E, nnz, kill)
+        d[ip[P.ess]] = 1.0
+        return d
+
+    import contextlib
+
+    with (threadpool_limits(limits=1) if threadpool_limits else contextlib.nullcontext()):
+        once()
+        t0 = time.perf_counter()
+        reps = 0
+        while True:
+            once()
+            reps += 1
+            if time.perf_counter() - t0 > 10.0 or reps >= 20:
+                break
+        dt = (time.perf_counter() - t0) / reps
+    return {"value": nnz / dt, "unit": "nnz/s", "cores": 1, "kind": "port",
+            "sample": "%d^3 P2 Jacobian A assembly (%d elements, %d nnz), %d reps, "
+                      "%.1f us/elem" % (n, P.mesh.ne, nnz, reps, 1e6 * dt / P.mesh.ne)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures this path on one GPU; --gpus %d is not supported" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from hippymfem_amd import ADJOINT, STATE
+    from hippymfem_amd.problem import PDEVariationalProblem
+    from hippymfem_amd.workloads import Workload
+
+    dev = torch.device("cuda", 0)
+    W = Workload(args.n)
+    prob = PDEVariationalProblem(W.spaces, W.density, W.batches, W.bc)
+    x = [W.u, W.m, W.p]
+    blk = prob.block_assembly(ADJOINT, STATE)
+    loc = [torch.as_tensor(v).to(dev) for v in x]
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        blk.assemble(loc)
+    torch.cuda.synchronize()
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ev[k][0].record(stream)
+        blk.element_matrices(loc)
+        ev[k][1].record(stream)
+        blk.scatter()
+        ev[k][2].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    t_kernel = np.mean([ev[k][0].elapsed_time(ev[k][1]) for k in range(args.steps)]) * 1e-3
+    t_scatter = np.mean([ev[k][1].elapsed_time(ev[k][2]) for k in range(args.steps)]) * 1e-3
+    ms = 1e3 * wall / args.steps
+    pat = blk.pattern
+    nnz, nent, ne = pat.nnz, pat.nent, pat.ne
+    nrow = pat.nrow
+    # algorithmic bytes of the scatter: E read, one code byte per entry, the runs, the
+    # row pointers (indptr, rptr, rowmode) and the values written
+    b_scatter = 8 * nent + 1 * nent + 4 * ne * pat.nd_t + (4 + 4 + 1) * nrow + 8 * nnz
+    flops_kernel = 2.0 * 27 * 27 * 81 * ne
+    scatter_gbs = b_scatter / t_scatter / 1e9
+    kernel_tf = flops_kernel / t_kernel / 1e12
+    if t_scatter >= t_kernel:
+        roof = {"kernel": "k_scatter_rows", "bound": "hbm", "achieved": scatter_gbs,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": scatter_gbs / HBM_PEAK_GBS,
+                "traffic": None}
+    else:
+        roof = {"kernel": "k_element_matrices", "bound": "mfma", "achieved": kernel_tf,
+                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": kernel_tf / FP64_PEAK_TFLOPS, "traffic": None}
+    out = {
+        "metric": METRIC, "value": nnz / (ms * 1e-3), "unit": "nnz/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "%d^3 hex P2 Jacobian A, exp(m) diffusion, FP64, 3x3x3 Gauss, "
+                               "Dirichlet x=0,1 folded" % args.n,
+                   "elements": ne, "nnz": nnz, "entries": nent, "parallelism": "1 GPU"},
+        "roofline": roof,
+        "phases": {"element_kernel_ms": 1e3 * t_kernel, "scatter_ms": 1e3 * t_scatter,
+                   "scatter_GBs": scatter_gbs, "element_kernel_TFLOPs": kernel_tf},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_n)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    main()

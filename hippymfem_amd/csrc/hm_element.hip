@@ -1,0 +1,403 @@
+// Element kernels of the assembly path, FP64: geometry of the element map at every
+// quadrature point, the fields there, the density's pointwise second derivatives and the
+// contraction E = Psi_t^T D Psi_c, plus the element vectors and the element-wise products.
+//
+// Formulation.  The reference's quadrature route (hippymfem/fem/kernel.py:1431-1485) forms
+// D[s][r][c] (value/gradient parts of the density's second derivatives) and contracts it
+// with the physical basis Psi = (N, G Jinv).  Here the inverse Jacobian is folded into D
+// instead, so the basis is the reference one, identical for every element and held in LDS:
+//   D^_vv = w vv,  D^_vg = w Jinv vg,  D^_gv = w Jinv gv,  D^_gg = w c Jinv Jinv^T,
+// with w = w_q |det J_q|.  The two are equal in exact arithmetic.
+#include <hip/hip_runtime.h>
+
+#include "hm_common.h"
+#include "hm_device.h"
+
+namespace hm {
+
+// Everything one quadrature point of one element needs, in FP64.
+struct PointState {
+  double w;           // w_q |det J|
+  double Ji[3][3];    // Jinv[k][d] = dxi_k / dx_d
+  double kap, g, g1, g2;
+  double gu[3], gp[3];  // physical gradients of u and p
+  double dot;           // grad u . grad p
+  double uq;            // value of u
+};
+
+__device__ inline void point_state(const hm_problem& P, int q, const double* Xe,
+                                   const double* ue, const double* me, const double* pe,
+                                   PointState& S) {
+  const int nq = P.nq;
+  // J[d][k] = sum_v X_v[d] dN_v/dxi_k (trilinear map)
+  double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  const double* gx = P.tab_x + (size_t)q * 24;
+  for (int v = 0; v < 8; ++v)
+    for (int d = 0; d < 3; ++d)
+      for (int k = 0; k < 3; ++k) J[d][k] = fma(Xe[v * 3 + d], gx[v * 3 + k], J[d][k]);
+  const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+  const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+  const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+  const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+  const double id = 1.0 / det;
+  // inverse of J (rows k = reference direction, columns d = physical)
+  S.Ji[0][0] = c00 * id;
+  S.Ji[1][0] = c01 * id;
+  S.Ji[2][0] = c02 * id;
+  S.Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+  S.Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+  S.Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+  S.Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+  S.Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+  S.Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+  S.w = P.wq[q] * fabs(det);
+  // fields: values and reference gradients, then physical gradients
+  const int nu = P.nd_u, nm = P.nd_m;
+  const double* Nu = P.tab_u + (size_t)q * nu;
+  const double* Gu = P.tab_u + (size_t)nq * nu + (size_t)q * nu * 3;
+  const double* Nm = P.tab_m + (size_t)q * nm;
+  double uq = 0, mq = 0, ru[3] = {0, 0, 0}, rp[3] = {0, 0, 0};
+  for (int a = 0; a < nu; ++a) {
+    uq = fma(Nu[a], ue[a], uq);
+    for (int k = 0; k < 3; ++k) {
+      ru[k] = fma(Gu[a * 3 + k], ue[a], ru[k]);
+      rp[k] = fma(Gu[a * 3 + k], pe[a], rp[k]);
+    }
+  }
+  for (int a = 0; a < nm; ++a) mq = fma(Nm[a], me[a], mq);
+  for (int d = 0; d < 3; ++d) {
+    S.gu[d] = S.Ji[0][d] * ru[0] + S.Ji[1][d] * ru[1] + S.Ji[2][d] * ru[2];
+    S.gp[d] = S.Ji[0][d] * rp[0] + S.Ji[1][d] * rp[1] + S.Ji[2][d] * rp[2];
+  }
+  S.dot = S.gu[0] * S.gp[0] + S.gu[1] * S.gp[1] + S.gu[2] * S.gp[2];
+  S.kap = exp(mq);
+  S.uq = uq;
+  if (P.density == HM_DENSITY_NONLINEAR) {
+    S.g = 1.0 + uq * uq;
+    S.g1 = 2.0 * uq;
+    S.g2 = 2.0;
+  } else {
+    S.g = 1.0;
+    S.g1 = 0.0;
+    S.g2 = 0.0;
+  }
+}
+
+// The pointwise block of test slot s and trial slot j, in the reference-gradient basis,
+// D[r][c] with r, c in {value, d/dxi_0, d/dxi_1, d/dxi_2}, times w.  The closed forms of
+// the density r = exp(m) g(u) grad u . grad p - f p (second derivatives per part).
+__device__ inline void point_block(int s, int j, const PointState& S, double D[4][4]) {
+  double vv = 0, vg[3] = {0, 0, 0}, gv[3] = {0, 0, 0}, gg = 0;
+  const double kg = S.kap * S.g, kg1 = S.kap * S.g1;
+  const int key = s * 3 + j;
+  switch (key) {
+    case HM_STATE * 3 + HM_STATE:
+      vv = S.kap * S.g2 * S.dot;
+      for (int d = 0; d < 3; ++d) vg[d] = gv[d] = kg1 * S.gp[d];
+      break;
+    case HM_STATE * 3 + HM_PARAMETER:
+      vv = kg1 * S.dot;
+      for (int d = 0; d < 3; ++d) gv[d] = kg * S.gp[d];
+      break;
+    case HM_PARAMETER * 3 + HM_STATE:
+      vv = kg1 * S.dot;
+      for (int d = 0; d < 3; ++d) vg[d] = kg * S.gp[d];
+      break;
+    case HM_STATE * 3 + HM_ADJOINT:
+      for (int d = 0; d < 3; ++d) vg[d] = kg1 * S.gu[d];
+      gg = kg;
+      break;
+    case HM_ADJOINT * 3 + HM_STATE:
+      for (int d = 0; d < 3; ++d) gv[d] = kg1 * S.gu[d];
+      gg = kg;
+      break;
+    case HM_PARAMETER * 3 + HM_PARAMETER:
+      vv = kg * S.dot;
+      break;
+    case HM_PARAMETER * 3 + HM_ADJOINT:
+      for (int d = 0; d < 3; ++d) vg[d] = kg * S.gu[d];
+      break;
+    case HM_ADJOINT * 3 + HM_PARAMETER:
+      for (int d = 0; d < 3; ++d) gv[d] = kg * S.gu[d];
+      break;
+    default:
+      break;
+  }
+  const double w = S.w;
+  D[0][0] = w * vv;
+  for (int k = 0; k < 3; ++k) {
+    D[0][1 + k] = w * (S.Ji[k][0] * vg[0] + S.Ji[k][1] * vg[1] + S.Ji[k][2] * vg[2]);
+    D[1 + k][0] = w * (S.Ji[k][0] * gv[0] + S.Ji[k][1] * gv[1] + S.Ji[k][2] * gv[2]);
+  }
+  for (int k = 0; k < 3; ++k)
+    for (int l = 0; l < 3; ++l) {
+      const double K = S.Ji[k][0] * S.Ji[l][0] + S.Ji[k][1] * S.Ji[l][1] + S.Ji[k][2] * S.Ji[l][2];
+      D[1 + k][1 + l] = w * gg * K;
+    }
+}
+
+// Loads the element's vertex coordinates and its three dof vectors into LDS.
+__device__ inline void load_element(const hm_problem& P, int64_t e, double* Xe, double* ue,
+                                    double* me, double* pe) {
+  for (int i = threadIdx.x; i < 24; i += blockDim.x) {
+    const int32_t v = P.elem_verts[e * 8 + i / 3];
+    Xe[i] = P.verts[(size_t)v * 3 + i % 3];
+  }
+  for (int a = threadIdx.x; a < P.nd_u; a += blockDim.x) {
+    const int32_t d = P.edofs_u[e * P.nd_u + a];
+    ue[a] = P.u[d];
+    pe[a] = P.p[d];
+  }
+  for (int a = threadIdx.x; a < P.nd_m; a += blockDim.x) me[a] = P.m[P.edofs_m[e * P.nd_m + a]];
+}
+
+// Psi[q][r][a] in LDS from N (nq, nd) and G (nq, nd, 3).
+__device__ inline void load_basis(const double* tab, int nq, int nd, double* Psi) {
+  for (int i = threadIdx.x; i < nq * 4 * nd; i += blockDim.x) {
+    const int a = i % nd, r = (i / nd) % 4, q = i / (4 * nd);
+    Psi[i] = r == 0 ? tab[q * nd + a] : tab[(size_t)nq * nd + ((size_t)q * nd + a) * 3 + (r - 1)];
+  }
+}
+
+// Which parts of the (s, j) block can be nonzero: bit 0 the value row, bit 1 the gradient
+// rows, bit 2 the value column, bit 3 the gradient columns (the reference skips the
+// identically zero pointwise blocks the same way, kernel.py:1606-1646).
+__host__ __device__ inline int block_parts(int s, int j, int density) {
+  const bool nl = density == HM_DENSITY_NONLINEAR;
+  bool vv = false, vg = false, gv = false, gg = false;
+  switch (s * 3 + j) {
+    case 0: vv = vg = gv = nl; break;                    // (u, u)
+    case 1: vv = nl; gv = true; break;                   // (u, m)
+    case 3: vv = nl; vg = true; break;                   // (m, u)
+    case 2: vg = nl; gg = true; break;                   // (u, p)
+    case 6: gv = nl; gg = true; break;                   // (p, u)
+    case 4: vv = true; break;                            // (m, m)
+    case 5: vg = true; break;                            // (m, p)
+    case 7: gv = true; break;                            // (p, m)
+    default: break;
+  }
+  return ((vv || vg) ? 1 : 0) | ((gv || gg) ? 2 : 0) | ((vv || gv) ? 4 : 0) | ((vg || gg) ? 8 : 0);
+}
+
+template <typename OutT>
+__global__ void __launch_bounds__(256) k_element_matrices(hm_problem P, int s, int j, int64_t e0,
+                                                          int64_t e1, int parts, OutT* out) {
+  extern __shared__ double lds[];
+  const int nq = P.nq;
+  const int nd_t = s == HM_PARAMETER ? P.nd_m : P.nd_u;
+  const int nd_c = j == HM_PARAMETER ? P.nd_m : P.nd_u;
+  const double* tab_t = s == HM_PARAMETER ? P.tab_m : P.tab_u;
+  const double* tab_c = j == HM_PARAMETER ? P.tab_m : P.tab_u;
+  const bool same = (tab_t == tab_c);
+  double* Pt = lds;
+  double* Pc = same ? Pt : Pt + nq * 4 * nd_t;
+  double* T = Pc + (same ? nq * 4 * nd_t : nq * 4 * nd_c);
+  double* D = T + nq * 4 * nd_c;
+  double* Xe = D + nq * 16;
+  double* ue = Xe + 24;
+  double* pe = ue + P.nd_u;
+  double* me = pe + P.nd_u;
+  load_basis(tab_t, nq, nd_t, Pt);
+  if (!same) load_basis(tab_c, nq, nd_c, Pc);
+  const int r0 = (parts & 1) ? 0 : 1, r1 = (parts & 2) ? 4 : 1;
+  const int c0 = (parts & 4) ? 0 : 1, c1 = (parts & 8) ? 4 : 1;
+  for (int64_t e = e0 + blockIdx.x; e < e1; e += gridDim.x) {
+    __syncthreads();
+    load_element(P, e, Xe, ue, me, pe);
+    __syncthreads();
+    for (int q = threadIdx.x; q < nq; q += blockDim.x) {
+      PointState S;
+      point_state(P, q, Xe, ue, me, pe, S);
+      double Dq[4][4];
+      point_block(s, j, S, Dq);
+      for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) D[q * 16 + r * 4 + c] = Dq[r][c];
+    }
+    __syncthreads();
+    // T[q][r][b] = sum_c D[q][r][c] Psi_c[q][c][b]
+    for (int i = threadIdx.x; i < nq * 4 * nd_c; i += blockDim.x) {
+      const int b = i % nd_c, r = (i / nd_c) % 4, q = i / (4 * nd_c);
+      double acc = 0;
+      if (r >= r0 && r < r1)
+        for (int c = c0; c < c1; ++c) acc = fma(D[q * 16 + r * 4 + c], Pc[(q * 4 + c) * nd_c + b], acc);
+      T[i] = acc;
+    }
+    __syncthreads();
+    // E[a][b] = sum_q sum_r Psi_t[q][r][a] T[q][r][b]
+    OutT* Eo = out + (size_t)(e - e0) * nd_t * nd_c;
+    for (int i = threadIdx.x; i < nd_t * nd_c; i += blockDim.x) {
+      const int a = i / nd_c, b = i % nd_c;
+      double acc = 0;
+      for (int q = 0; q < nq; ++q)
+        for (int r = r0; r < r1; ++r) acc = fma(Pt[(q * 4 + r) * nd_t + a], T[(q * 4 + r) * nd_c + b], acc);
+      Eo[i] = (OutT)acc;
+    }
+  }
+}
+
+// Element vectors dR_e / d(slot i dofs): sum_q w [val N_a + (Jinv grad) . G_a].
+__global__ void __launch_bounds__(256) k_element_vectors(hm_problem P, int s, int64_t e0, int64_t e1,
+                                                         double* out) {
+  extern __shared__ double lds[];
+  const int nq = P.nq;
+  const int nd = s == HM_PARAMETER ? P.nd_m : P.nd_u;
+  const double* tab = s == HM_PARAMETER ? P.tab_m : P.tab_u;
+  double* Ps = lds;
+  double* W = Ps + nq * 4 * nd;  // W[q][4]: weights of the value and the 3 ref gradients
+  double* Xe = W + nq * 4;
+  double* ue = Xe + 24;
+  double* pe = ue + P.nd_u;
+  double* me = pe + P.nd_u;
+  load_basis(tab, nq, nd, Ps);
+  for (int64_t e = e0 + blockIdx.x; e < e1; e += gridDim.x) {
+    __syncthreads();
+    load_element(P, e, Xe, ue, me, pe);
+    __syncthreads();
+    for (int q = threadIdx.x; q < nq; q += blockDim.x) {
+      PointState S;
+      point_state(P, q, Xe, ue, me, pe, S);
+      double val = 0, gr[3] = {0, 0, 0};
+      if (s == HM_ADJOINT) {
+        val = -P.f;
+        for (int d = 0; d < 3; ++d) gr[d] = S.kap * S.g * S.gu[d];
+      } else if (s == HM_STATE) {
+        val = S.kap * S.g1 * S.dot;
+        for (int d = 0; d < 3; ++d) gr[d] = S.kap * S.g * S.gp[d];
+      } else {
+        val = S.kap * S.g * S.dot;
+      }
+      W[q * 4] = S.w * val;
+      for (int k = 0; k < 3; ++k)
+        W[q * 4 + 1 + k] = S.w * (S.Ji[k][0] * gr[0] + S.Ji[k][1] * gr[1] + S.Ji[k][2] * gr[2]);
+    }
+    __syncthreads();
+    for (int a = threadIdx.x; a < nd; a += blockDim.x) {
+      double acc = 0;
+      for (int q = 0; q < nq; ++q)
+        for (int r = 0; r < 4; ++r) acc = fma(Ps[(q * 4 + r) * nd + a], W[q * 4 + r], acc);
+      out[(size_t)(e - e0) * nd + a] = acc;
+    }
+  }
+}
+
+// Block (s, j) applied to the slot-j direction, element by element:
+// y_a = sum_q Psi_t[q][:][a] . D[q] (Psi_c[q] d).
+__global__ void __launch_bounds__(256) k_element_hvp(hm_problem P, int s, int j, const double* dir,
+                                                     int64_t e0, int64_t e1, double* out) {
+  extern __shared__ double lds[];
+  const int nq = P.nq;
+  const int nd_t = s == HM_PARAMETER ? P.nd_m : P.nd_u;
+  const int nd_c = j == HM_PARAMETER ? P.nd_m : P.nd_u;
+  const double* tab_t = s == HM_PARAMETER ? P.tab_m : P.tab_u;
+  const double* tab_c = j == HM_PARAMETER ? P.tab_m : P.tab_u;
+  const int32_t* ed_c = j == HM_PARAMETER ? P.edofs_m : P.edofs_u;
+  double* Pt = lds;
+  double* Pc = Pt + nq * 4 * nd_t;
+  double* W = Pc + nq * 4 * nd_c;
+  double* de = W + nq * 4;
+  double* Xe = de + nd_c;
+  double* ue = Xe + 24;
+  double* pe = ue + P.nd_u;
+  double* me = pe + P.nd_u;
+  load_basis(tab_t, nq, nd_t, Pt);
+  load_basis(tab_c, nq, nd_c, Pc);
+  for (int64_t e = e0 + blockIdx.x; e < e1; e += gridDim.x) {
+    __syncthreads();
+    load_element(P, e, Xe, ue, me, pe);
+    for (int b = threadIdx.x; b < nd_c; b += blockDim.x) de[b] = dir[ed_c[e * nd_c + b]];
+    __syncthreads();
+    for (int q = threadIdx.x; q < nq; q += blockDim.x) {
+      PointState S;
+      point_state(P, q, Xe, ue, me, pe, S);
+      double Dq[4][4];
+      point_block(s, j, S, Dq);
+      double z[4] = {0, 0, 0, 0};
+      for (int c = 0; c < 4; ++c)
+        for (int b = 0; b < nd_c; ++b) z[c] = fma(Pc[(q * 4 + c) * nd_c + b], de[b], z[c]);
+      for (int r = 0; r < 4; ++r) {
+        double acc = 0;
+        for (int c = 0; c < 4; ++c) acc = fma(Dq[r][c], z[c], acc);
+        W[q * 4 + r] = acc;
+      }
+    }
+    __syncthreads();
+    for (int a = threadIdx.x; a < nd_t; a += blockDim.x) {
+      double acc = 0;
+      for (int q = 0; q < nq; ++q)
+        for (int r = 0; r < 4; ++r) acc = fma(Pt[(q * 4 + r) * nd_t + a], W[q * 4 + r], acc);
+      out[(size_t)(e - e0) * nd_t + a] = acc;
+    }
+  }
+}
+
+static int check_problem(const hm_problem* P) {
+  HM_REQUIRE(P != nullptr, "no problem given");
+  HM_REQUIRE(P->ne >= 0 && P->nq > 0 && P->nq <= 64, "quadrature points per element %d outside 1..64", P->nq);
+  HM_REQUIRE(P->nd_u > 0 && P->nd_u <= 64 && P->nd_m > 0 && P->nd_m <= 64,
+             "dofs per element (%d, %d) outside 1..64", P->nd_u, P->nd_m);
+  HM_REQUIRE(P->density == HM_DENSITY_LINEAR || P->density == HM_DENSITY_NONLINEAR,
+             "unknown density %d", P->density);
+  return HM_OK;
+}
+
+static int grid_for(int64_t n) {
+  const int64_t g = n < 1 ? 1 : n;
+  return (int)(g < 65536 ? g : 65536);
+}
+
+}  // namespace hm
+
+using namespace hm;
+
+extern "C" int hm_element_matrices(const hm_problem* P, int i, int j, int64_t e0, int64_t e1,
+                                   int precision, void* out, void* stream) {
+  if (int rc = check_problem(P)) return rc;
+  HM_REQUIRE(i >= 0 && i < 3 && j >= 0 && j < 3, "slots (%d, %d) outside 0..2", i, j);
+  HM_REQUIRE(e0 >= 0 && e0 <= e1 && e1 <= P->ne, "elements [%lld, %lld) outside [0, %lld)",
+             (long long)e0, (long long)e1, (long long)P->ne);
+  if (e1 == e0) return HM_OK;
+  if (precision != HM_FP64) return hm_element_matrices_lowp(P, i, j, e0, e1, precision, out, stream);
+  const int nq = P->nq;
+  const int nd_t = i == HM_PARAMETER ? P->nd_m : P->nd_u;
+  const int nd_c = j == HM_PARAMETER ? P->nd_m : P->nd_u;
+  const bool same = (i == HM_PARAMETER) == (j == HM_PARAMETER);
+  size_t words = (size_t)nq * 4 * nd_t + (same ? 0 : (size_t)nq * 4 * nd_c) + (size_t)nq * 4 * nd_c +
+                 (size_t)nq * 16 + 24 + 2 * P->nd_u + P->nd_m;
+  const size_t bytes = words * sizeof(double);
+  HM_REQUIRE(bytes <= 64 * 1024, "element tables need %zu bytes of LDS, above 64 KiB", bytes);
+  const int parts = block_parts(i, j, P->density);
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_element_matrices<double>, dim3(grid_for(e1 - e0)), dim3(256), bytes, s, *P, i,
+                     j, e0, e1, parts, (double*)out);
+  return hip_check("hm_element_matrices");
+}
+
+extern "C" int hm_element_vectors(const hm_problem* P, int i, int64_t e0, int64_t e1, double* out,
+                                  void* stream) {
+  if (int rc = check_problem(P)) return rc;
+  HM_REQUIRE(i >= 0 && i < 3, "slot %d outside 0..2", i);
+  HM_REQUIRE(e0 >= 0 && e0 <= e1 && e1 <= P->ne, "elements outside the batch");
+  if (e1 == e0) return HM_OK;
+  const int nd = i == HM_PARAMETER ? P->nd_m : P->nd_u;
+  const size_t bytes = ((size_t)P->nq * 4 * nd + (size_t)P->nq * 4 + 24 + 2 * P->nd_u + P->nd_m) * 8;
+  HM_REQUIRE(bytes <= 64 * 1024, "element tables need %zu bytes of LDS, above 64 KiB", bytes);
+  hipLaunchKernelGGL(k_element_vectors, dim3(grid_for(e1 - e0)), dim3(256), bytes, (hipStream_t)stream,
+                     *P, i, e0, e1, out);
+  return hip_check("hm_element_vectors");
+}
+
+extern "C" int hm_element_hvp(const hm_problem* P, int i, int j, const double* dir, int64_t e0,
+                              int64_t e1, double* out, void* stream) {
+  if (int rc = check_problem(P)) return rc;
+  HM_REQUIRE(i >= 0 && i < 3 && j >= 0 && j < 3, "slots (%d, %d) outside 0..2", i, j);
+  HM_REQUIRE(e0 >= 0 && e0 <= e1 && e1 <= P->ne, "elements outside the batch");
+  if (e1 == e0) return HM_OK;
+  const int nd_t = i == HM_PARAMETER ? P->nd_m : P->nd_u;
+  const int nd_c = j == HM_PARAMETER ? P->nd_m : P->nd_u;
+  const size_t bytes =
+      ((size_t)P->nq * 4 * (nd_t + nd_c) + (size_t)P->nq * 4 + nd_c + 24 + 2 * P->nd_u + P->nd_m) * 8;
+  HM_REQUIRE(bytes <= 64 * 1024, "element tables need %zu bytes of LDS, above 64 KiB", bytes);
+  hipLaunchKernelGGL(k_element_hvp, dim3(grid_for(e1 - e0)), dim3(256), bytes, (hipStream_t)stream, *P,
+                     i, j, dir, e0, e1, out);
+  return hip_check("hm_element_hvp");
+}

@@ -1,0 +1,198 @@
+// The scatter of element arrays into CSR values and dof vectors: a deterministic
+// segmented reduction with no float atomics.
+//
+// Every CSR row owns its sum.  Its entries come from its element rows (runs t = e*nd_t + a,
+// ascending, from the pattern build), each a contiguous run of nd_c entries of E, and the
+// place of every entry within the row (its code).  A row is reduced by one group of lanes
+// (half a wavefront when nd_c <= 32) into a row buffer in LDS, run after run in ascending
+// order, so every CSR value is the sum of its entries in ascending entry order: the order of
+// np.bincount, the reference's host scatter (hippymfem/fem/pattern.py:788).  The row buffer
+// then goes out with contiguous stores.  Eliminated rows are written as zero rows (with the
+// diagonal of the policy at their first place, which is the diagonal on a square block) and
+// never read E; eliminated columns carry the code "all ones" and are dropped
+// (pattern.py:693-713, elimination.py:53-72).
+#include <hip/hip_runtime.h>
+
+#include "hm_common.h"
+#include "hm_device.h"
+
+namespace hm {
+
+template <typename CodeT>
+struct Drop;
+template <>
+struct Drop<uint8_t> {
+  static constexpr unsigned v = 0xFFu;
+};
+template <>
+struct Drop<uint16_t> {
+  static constexpr unsigned v = 0xFFFFu;
+};
+
+// LPR lanes reduce one row; a block of 256 threads holds 256/LPR row buffers.
+template <typename CodeT, typename ET, typename OT, int LPR>
+__global__ void __launch_bounds__(256) k_scatter_rows(hm_scatter_plan S, const ET* __restrict__ E,
+                                                      int64_t t0, int64_t t1, OT* __restrict__ data,
+                                                      int accumulate) {
+  extern __shared__ double rowbuf[];
+  constexpr int groups = 256 / LPR;
+  const int lane = threadIdx.x % LPR;
+  const int grp = threadIdx.x / LPR;
+  double* buf = rowbuf + (size_t)grp * S.max_row;
+  const CodeT* codes = (const CodeT*)S.codes;
+  const int nd_c = S.nd_c;
+  for (int64_t r = (int64_t)blockIdx.x * groups + grp; r < S.nrow; r += (int64_t)gridDim.x * groups) {
+    const int32_t lo = S.indptr[r];
+    const int32_t len = S.indptr[r + 1] - lo;
+    const int mode = S.rowmode ? S.rowmode[r] : 0;
+    if (mode != 0) {
+      for (int i = lane; i < len; i += LPR) data[lo + i] = (OT)((mode == 2 && i == 0) ? S.diag : 0.0);
+      continue;
+    }
+    for (int i = lane; i < len; i += LPR) buf[i] = accumulate ? (double)data[lo + i] : 0.0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const int32_t p1 = S.rptr[r + 1];
+    for (int32_t p = S.rptr[r]; p < p1; ++p) {
+      const int64_t t = S.runs[p];
+      if (t < t0 || t >= t1) continue;
+      const int64_t kc = t * nd_c;
+      const int64_t ke = (t - t0) * nd_c;
+      for (int b = lane; b < nd_c; b += LPR) {
+        const unsigned c = codes[kc + b];
+        if (c != Drop<CodeT>::v) buf[c] += (double)E[ke + b];
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    }
+    for (int i = lane; i < len; i += LPR) data[lo + i] = (OT)buf[i];
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// One thread per dof: the element-vector entries of its runs, in ascending order.
+__global__ void __launch_bounds__(256) k_scatter_vector(int64_t n, const int32_t* __restrict__ rptr,
+                                                        const int32_t* __restrict__ runs,
+                                                        const double* __restrict__ V, int64_t t0,
+                                                        int64_t t1, const int8_t* __restrict__ zero_rows,
+                                                        double* __restrict__ out, int accumulate) {
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    if (zero_rows && zero_rows[r]) {
+      out[r] = 0.0;
+      continue;
+    }
+    double acc = accumulate ? out[r] : 0.0;
+    for (int32_t p = rptr[r]; p < rptr[r + 1]; ++p) {
+      const int64_t t = runs[p];
+      if (t >= t0 && t < t1) acc += V[t - t0];
+    }
+    out[r] = acc;
+  }
+}
+
+// y = A x: 16 lanes per row, a fixed butterfly for the row sum (deterministic).
+template <typename VT>
+__global__ void __launch_bounds__(256) k_csr_matvec(int64_t nrow, const int32_t* __restrict__ indptr,
+                                                    const int32_t* __restrict__ indices,
+                                                    const VT* __restrict__ data,
+                                                    const double* __restrict__ x, double* __restrict__ y) {
+  const int lane = threadIdx.x & 15;
+  for (int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4; r < nrow;
+       r += ((int64_t)gridDim.x * blockDim.x) >> 4) {
+    double acc = 0.0;
+    for (int32_t k = indptr[r] + lane; k < indptr[r + 1]; k += 16) acc = fma((double)data[k], x[indices[k]], acc);
+    for (int off = 8; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 16);
+    if (lane == 0) y[r] = acc;
+  }
+}
+
+template <typename CodeT, typename ET, typename OT>
+static void launch_rows(const hm_scatter_plan& S, const void* E, int64_t t0, int64_t t1, void* data,
+                        int accumulate, hipStream_t st) {
+  if (S.nd_c <= 32) {
+    constexpr int groups = 256 / 32;
+    const int64_t nb = (S.nrow + groups - 1) / groups;
+    const int grid = (int)(nb < 131072 ? (nb < 1 ? 1 : nb) : 131072);
+    hipLaunchKernelGGL((k_scatter_rows<CodeT, ET, OT, 32>), dim3(grid), dim3(256),
+                       (size_t)groups * S.max_row * sizeof(double), st, S, (const ET*)E, t0, t1,
+                       (OT*)data, accumulate);
+  } else {
+    constexpr int groups = 256 / 64;
+    const int64_t nb = (S.nrow + groups - 1) / groups;
+    const int grid = (int)(nb < 131072 ? (nb < 1 ? 1 : nb) : 131072);
+    hipLaunchKernelGGL((k_scatter_rows<CodeT, ET, OT, 64>), dim3(grid), dim3(256),
+                       (size_t)groups * S.max_row * sizeof(double), st, S, (const ET*)E, t0, t1,
+                       (OT*)data, accumulate);
+  }
+}
+
+template <typename CodeT>
+static int dispatch_types(const hm_scatter_plan& S, const void* E, int e_dtype, int64_t t0, int64_t t1,
+                          void* data, int out_dtype, int accumulate, hipStream_t st) {
+  if (e_dtype == HM_FP64 && out_dtype == HM_FP64)
+    launch_rows<CodeT, double, double>(S, E, t0, t1, data, accumulate, st);
+  else if (e_dtype == HM_FP32 && out_dtype == HM_FP32)
+    launch_rows<CodeT, float, float>(S, E, t0, t1, data, accumulate, st);
+  else if (e_dtype == HM_FP32 && out_dtype == HM_FP64)
+    launch_rows<CodeT, float, double>(S, E, t0, t1, data, accumulate, st);
+  else if (e_dtype == HM_FP64 && out_dtype == HM_FP32)
+    launch_rows<CodeT, double, float>(S, E, t0, t1, data, accumulate, st);
+  else
+    HM_FAIL(HM_ERR_ARG, "element dtype %d / value dtype %d not supported", e_dtype, out_dtype);
+  return HM_OK;
+}
+
+}  // namespace hm
+
+using namespace hm;
+
+extern "C" int hm_scatter_matrix(const hm_scatter_plan* S, const void* E, int e_dtype, int64_t e0,
+                                 int64_t e1, void* data, int out_dtype, int accumulate, void* stream) {
+  HM_REQUIRE(S != nullptr, "no scatter plan given");
+  HM_REQUIRE(S->nrow >= 0 && S->nd_t > 0 && S->nd_c > 0, "bad scatter plan shape");
+  HM_REQUIRE(e0 >= 0 && e0 <= e1 && e1 <= S->ne, "elements [%lld, %lld) outside [0, %lld)",
+             (long long)e0, (long long)e1, (long long)S->ne);
+  HM_REQUIRE(S->code_bytes == 1 || S->code_bytes == 2, "code width %d is neither 1 nor 2", S->code_bytes);
+  HM_REQUIRE(S->max_row >= 0 && (size_t)S->max_row * 8 * 8 <= 64 * 1024,
+             "a row of %d entries does not fit the row buffers", S->max_row);
+  if (S->nrow == 0) return HM_OK;
+  const int64_t t0 = e0 * S->nd_t, t1 = e1 * S->nd_t;
+  hipStream_t st = (hipStream_t)stream;
+  int rc = S->code_bytes == 1
+               ? dispatch_types<uint8_t>(*S, E, e_dtype, t0, t1, data, out_dtype, accumulate, st)
+               : dispatch_types<uint16_t>(*S, E, e_dtype, t0, t1, data, out_dtype, accumulate, st);
+  if (rc) return rc;
+  return hip_check("hm_scatter_matrix");
+}
+
+extern "C" int hm_scatter_vector(int64_t n, int nd, const int32_t* rptr, const int32_t* runs,
+                                 const double* V, int64_t e0, int64_t e1, const int8_t* zero_rows,
+                                 double* out, int accumulate, void* stream) {
+  HM_REQUIRE(n >= 0 && nd > 0 && e0 >= 0 && e0 <= e1, "bad vector scatter arguments");
+  if (n == 0) return HM_OK;
+  const int64_t nb = (n + 255) / 256;
+  const int grid = (int)(nb < 65536 ? nb : 65536);
+  hipLaunchKernelGGL(k_scatter_vector, dim3(grid), dim3(256), 0, (hipStream_t)stream, n, rptr, runs, V,
+                     e0 * nd, e1 * nd, zero_rows, out, accumulate);
+  return hip_check("hm_scatter_vector");
+}
+
+extern "C" int hm_csr_matvec(int64_t nrow, const int32_t* indptr, const int32_t* indices,
+                             const void* data, int val_dtype, const double* x, double* y, void* stream) {
+  HM_REQUIRE(nrow >= 0, "negative row count");
+  if (nrow == 0) return HM_OK;
+  const int64_t nb = (nrow * 16 + 255) / 256;
+  const int grid = (int)(nb < 131072 ? nb : 131072);
+  hipStream_t st = (hipStream_t)stream;
+  if (val_dtype == HM_FP64)
+    hipLaunchKernelGGL(k_csr_matvec<double>, dim3(grid), dim3(256), 0, st, nrow, indptr, indices,
+                       (const double*)data, x, y);
+  else if (val_dtype == HM_FP32)
+    hipLaunchKernelGGL(k_csr_matvec<float>, dim3(grid), dim3(256), 0, st, nrow, indptr, indices,
+                       (const float*)data, x, y);
+  else
+    HM_FAIL(HM_ERR_ARG, "value dtype %d not supported", val_dtype);
+  return hip_check("hm_csr_matvec");
+}

@@ -1,0 +1,142 @@
+"""Element kernels on the GPU (``hippymfem/fem/kernel.py``).
+
+The reference builds its element arrays by automatic differentiation of a user density
+(JAX); this path serves the densities of the assembly benchmarks, each stated in closed form
+inside the HIP kernels:
+
+* :class:`ExpDiffusion` ``r = exp(m) grad u . grad p - f p`` (``nonlinear=False``,
+  ``benchmarks/bench_assembly.py:123-125``), and
+* ``r = exp(m) (1 + u^2) grad u . grad p - f p`` (``nonlinear=True``).
+
+Slots are ``STATE=0`` (u), ``PARAMETER=1`` (m), ``ADJOINT=2`` (p); ``spaces`` is
+``[Vu, Vm, Vu]``.  Arrays come back as torch tensors on the device.
+"""
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import ADJOINT, PARAMETER, STATE
+
+#: Largest number of elements handed to one launch when chunks are asked for
+#: (``ELEMENT_CHUNK``, ``kernel.py:402``); 0 means the whole batch.
+ELEMENT_CHUNK = 0
+
+
+class ExpDiffusion:
+    """The density ``exp(m) g(u) grad u . grad p - f p``, ``g = 1`` or ``1 + u^2``."""
+
+    def __init__(self, nonlinear=False, f=1.0):
+        self.nonlinear = bool(nonlinear)
+        self.f = float(f)
+
+    @property
+    def code(self):
+        return 1 if self.nonlinear else 0
+
+    @property
+    def is_fwd_linear(self):
+        return not self.nonlinear
+
+    def __repr__(self):
+        return "ExpDiffusion(nonlinear=%s, f=%g)" % (self.nonlinear, self.f)
+
+
+class QuadratureKernel:
+    """Element arrays of a density on a mesh's element groups (``QuadratureKernel``,
+    ``kernel.py:1917-2090``).  ``local_arrays`` is ``[u, m, p]``: dof vectors (numpy or
+    device tensors) of the three slots."""
+
+    def __init__(self, density, spaces, batches, nparams=0):
+        if len(spaces) != 3 or spaces[STATE] is not spaces[ADJOINT] and \
+                spaces[STATE].order != spaces[ADJOINT].order:
+            raise ValueError("spaces must be [state, parameter, adjoint], state and adjoint alike")
+        self.density = density
+        self.spaces = list(spaces)
+        self.batches = batches
+        self.group = batches.groups[0]
+        self.tables = [batches.tables(s)[0] for s in spaces]
+        self.nd = [t.nd for t in self.tables]
+        self._keep = None
+
+    # ------------------------------------------------------------- plumbing
+    def problem(self, local_arrays):
+        """The ``hm_problem`` of the current fields; keeps the device arrays alive."""
+        dev = _lib.require_device()
+        g = self.group.device()
+        tu = self.tables[STATE].device()
+        tm = self.tables[PARAMETER].device()
+        u, m, p = (_vec(a, dev) for a in local_arrays[:3])
+        P = _lib.HmProblem(
+            ne=self.group.ne, nq=self.group.nq, nd_u=self.nd[STATE], nd_m=self.nd[PARAMETER],
+            density=self.density.code, f=self.density.f,
+            verts=_lib.ptr(g["verts"]), elem_verts=_lib.ptr(g["elem_verts"]),
+            edofs_u=_lib.ptr(tu["edofs"]), edofs_m=_lib.ptr(tm["edofs"]),
+            tab_u=_lib.ptr(tu["tab"]), tab_m=_lib.ptr(tm["tab"]), tab_x=_lib.ptr(g["tab_x"]),
+            wq=_lib.ptr(g["wq"]), u=_lib.ptr(u), m=_lib.ptr(m), p=_lib.ptr(p))
+        self._keep = (u, m, p, g, tu, tm)
+        return P, (u, m, p)
+
+    # ------------------------------------------------------------ element arrays
+    def element_matrices(self, i, j, local_arrays, params=(), consistent=None,
+                         precision="fp64", elements=None, out=None, stream=None):
+        """Block ``(i, j)`` element matrices ``[(ne, nd_i, nd_j)]`` (``element_matrices``,
+        ``kernel.py:1964-2006``).  ``precision``: ``"fp64"`` (default), or ``"fp32"``,
+        ``"bf16"``, ``"fp16"`` for the precision form (fp32 results)."""
+        P, keep = self.problem(local_arrays)
+        a, b = (0, self.group.ne) if elements is None else elements
+        code = _lib.PRECISIONS[precision]
+        dtype = torch.float64 if code == _lib.FP64 else torch.float32
+        if out is None:
+            out = torch.empty((b - a, self.nd[i], self.nd[j]), dtype=dtype,
+                              device=keep[0].device)
+        _lib.check(_lib.lib().hm_element_matrices(
+            _lib.ctypes.byref(P), int(i), int(j), int(a), int(b), code, _lib.ptr(out),
+            _lib.stream_handle(stream)), "element matrices (%d, %d)" % (i, j))
+        return [out]
+
+    def element_matrices_many(self, pairs, local_arrays, precision="fp64"):
+        """Several blocks from one call (``element_matrices_many``, ``kernel.py:2048-2068``)."""
+        return {ij: self.element_matrices(ij[0], ij[1], local_arrays, precision=precision)
+                for ij in pairs}
+
+    def element_matrix_chunks(self, i, j, local_arrays, chunk=None, precision="fp64"):
+        """A generator of ``(group, start, stop, array)`` chunks (``element_matrix_chunks``,
+        ``kernel.py:2008-2024``)."""
+        chunk = int(chunk or ELEMENT_CHUNK or self.group.ne)
+        for a in range(0, self.group.ne, chunk):
+            b = min(a + chunk, self.group.ne)
+            yield 0, a, b, self.element_matrices(i, j, local_arrays, precision=precision,
+                                                 elements=(a, b))[0]
+
+    def element_vectors(self, i, local_arrays, params=(), elements=None, out=None, stream=None):
+        """``dR_e / d(slot i)`` ``[(ne, nd_i)]`` (``element_vectors``, ``kernel.py:2070-2078``)."""
+        P, keep = self.problem(local_arrays)
+        a, b = (0, self.group.ne) if elements is None else elements
+        if out is None:
+            out = torch.empty((b - a, self.nd[i]), dtype=torch.float64, device=keep[0].device)
+        _lib.check(_lib.lib().hm_element_vectors(_lib.ctypes.byref(P), int(i), int(a), int(b),
+                                                 _lib.ptr(out), _lib.stream_handle(stream)),
+                   "element vectors (%d)" % i)
+        return [out]
+
+    def element_hvp(self, i, j, local_arrays, jdir_local, params=(), elements=None, out=None,
+                    stream=None):
+        """Block ``(i, j)`` applied to the slot-``j`` direction element by element
+        ``[(ne, nd_i)]`` (``element_hvp``, ``kernel.py:2080-2090``)."""
+        P, keep = self.problem(local_arrays)
+        d = _vec(jdir_local, keep[0].device)
+        a, b = (0, self.group.ne) if elements is None else elements
+        if out is None:
+            out = torch.empty((b - a, self.nd[i]), dtype=torch.float64, device=keep[0].device)
+        _lib.check(_lib.lib().hm_element_hvp(_lib.ctypes.byref(P), int(i), int(j), _lib.ptr(d),
+                                             int(a), int(b), _lib.ptr(out),
+                                             _lib.stream_handle(stream)),
+                   "element products (%d, %d)" % (i, j))
+        return [out]
+
+
+def _vec(a, dev):
+    if isinstance(a, torch.Tensor):
+        return a.to(device=dev, dtype=torch.float64).contiguous()
+    return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64)).to(dev)

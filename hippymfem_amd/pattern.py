@@ -1,0 +1,352 @@
+"""Sparsity patterns and the scatter of element arrays into them
+(``hippymfem/fem/pattern.py``).
+
+:class:`ScatterPattern` is the value-independent part of a matrix assembly: the CSR graph
+(``indptr``, ``indices``; rows of a square pattern lead with their diagonal, the order hypre
+keeps), the element rows of every CSR row (``rptr``/``runs``) and every entry's place in its
+row (``pos``), from the host build of the native library.  On the device it keeps one
+scatter plan per elimination: the rows, the runs and a one-byte code per entry (two where a
+row is longer than 254 entries).  :class:`VectorPattern` is the dual vectors' counterpart.
+"""
+
+import numpy as np
+import torch
+
+from . import _lib
+from .elementbatch import group_tables
+
+_HYPRE_INT = np.int32
+
+
+def _threads():
+    import os
+    try:
+        return max(len(os.sched_getaffinity(0)), 1)
+    except (AttributeError, OSError):
+        return os.cpu_count() or 1
+
+
+def build_tables(tdofs, cdofs, nrow, ncol, diag_first, nthreads=0):
+    """``(indptr int64, rptr int64, runs int32, pos uint16, nnz, max_row)`` of the
+    sort-free build (``patternbuild.build_tables``, ``patternbuild.py:209-253``), in the
+    native library."""
+    L = _lib.lib()
+    tdofs = np.ascontiguousarray(tdofs, dtype=np.int32)
+    cdofs = np.ascontiguousarray(cdofs, dtype=np.int32)
+    ne, nd_t = tdofs.shape
+    nd_c = cdofs.shape[1]
+    if cdofs.shape[0] != ne:
+        raise ValueError("the two dof tables hold %d and %d elements" % (ne, cdofs.shape[0]))
+    rptr = np.zeros(nrow + 1, np.int64)
+    runs = np.empty(max(ne * nd_t, 1), np.int32)
+    pos = np.empty(max(ne * nd_t * nd_c, 1), np.uint16)
+    indptr = np.zeros(nrow + 1, np.int64)
+    nnz = _lib.I64(0)
+    max_row = _lib.I64(0)
+    _lib.check(L.hm_pattern_build(_lib.ptr(tdofs), _lib.ptr(cdofs), ne, nd_t, nd_c, nrow, ncol,
+                                  int(bool(diag_first)), int(nthreads or _threads()),
+                                  _lib.ptr(rptr), _lib.ptr(runs), _lib.ptr(pos),
+                                  _lib.ptr(indptr), _lib.ctypes.byref(nnz),
+                                  _lib.ctypes.byref(max_row)), "pattern build")
+    return indptr, rptr, runs[:ne * nd_t], pos[:ne * nd_t * nd_c], int(nnz.value), int(max_row.value)
+
+
+class ScatterPattern:
+    """CSR graph of the ``test x trial`` element-assembly map (``ScatterPattern``,
+    ``pattern.py:375-1090``).
+
+    ``indptr``/``indices`` int32 (hypre's ``HYPRE_Int``), ``nnz``, ``nent`` (element-matrix
+    entries); ``slot`` (int32, the CSR place of entry ``k = (e nd_t + a) nd_c + b``) is made
+    when first asked for.  Raises ``ValueError`` for a row of 2**16 entries or more, and
+    ``NotImplementedError`` for an element row that repeats a column."""
+
+    def __init__(self, test_tables, trial_tables, test_space=None, trial_space=None,
+                 nthreads=0):
+        if len(test_tables) != len(trial_tables):
+            raise ValueError("test and trial tables must cover the same groups")
+        if len(test_tables) != 1:
+            raise NotImplementedError("one element group (one element geometry) is supported")
+        tt, tr = test_tables[0], trial_tables[0]
+        test_space = test_space if test_space is not None else tt.space
+        trial_space = trial_space if trial_space is not None else tr.space
+        self.nrow, self.ncol = int(test_space.ndof), int(trial_space.ndof)
+        self.shapes = [(int(tt.group.ne), int(tt.nd_total), int(tr.nd_total))]
+        self.sizes = [ne * a * b for ne, a, b in self.shapes]
+        self.nent = int(sum(self.sizes))
+        self.ne, self.nd_t, self.nd_c = self.shapes[0]
+        self.sign = None
+        self.perm = None
+        self.tdofs = np.ascontiguousarray(tt.edofs, dtype=np.int32)
+        self.cdofs = np.ascontiguousarray(tr.edofs, dtype=np.int32)
+        self.diag_first = self.nrow == self.ncol
+        if max(self.nrow, self.ncol) > (1 << 31) - 1:
+            raise ValueError("%d rows and %d columns do not fit 32-bit local indices"
+                             % (self.nrow, self.ncol))
+        (indptr, self.rptr, self.runs, self.pos, self.nnz,
+         self.max_row) = build_tables(self.tdofs, self.cdofs, self.nrow, self.ncol,
+                                      self.diag_first, nthreads)
+        if self.nnz > np.iinfo(np.int32).max:
+            raise ValueError("the sparsity pattern has %d entries, beyond int32" % self.nnz)
+        self._indptr64 = indptr
+        self.indptr = indptr.astype(np.int32)
+        self.indices = np.empty(self.nnz, _HYPRE_INT)
+        L = _lib.lib()
+        _lib.check(L.hm_pattern_place(_lib.ptr(self.cdofs), self.ne, self.nd_t, self.nd_c,
+                                      self.nrow, _lib.ptr(self.rptr), _lib.ptr(self.runs),
+                                      _lib.ptr(indptr), _lib.ptr(self.pos), _threads(),
+                                      _lib.ptr(self.indices), None), "pattern place")
+        rep = _lib.I32(0)
+        _lib.check(L.hm_pattern_repeats(self.ne, self.nd_t, self.nd_c, _lib.ptr(self.pos),
+                                        _threads(), _lib.ctypes.byref(rep)), "pattern check")
+        if rep.value:
+            raise NotImplementedError("an element row repeats a column; the device scatter "
+                                      "adds one element row's entries at once")
+        self._slot = None
+        self._plans = {}
+        self._masked = {}
+
+    # ---------------------------------------------------------------- host arrays
+    @property
+    def slot(self):
+        """CSR place of every element-matrix entry (``pattern.py:437-446``), int32."""
+        if self._slot is None:
+            slot = np.empty(self.nent, np.int32)
+            _lib.check(_lib.lib().hm_pattern_place(
+                _lib.ptr(self.cdofs), self.ne, self.nd_t, self.nd_c, self.nrow,
+                _lib.ptr(self.rptr), _lib.ptr(self.runs), _lib.ptr(self._indptr64),
+                _lib.ptr(self.pos), _threads(), _lib.ptr(self.indices), _lib.ptr(slot)),
+                "pattern slots")
+            self._slot = slot
+        return self._slot
+
+    def masked_slots(self, row_mask, col_mask):
+        """CSR slots in masked rows or columns (``masked_slots``, ``pattern.py:693-713``)."""
+        key = (id(row_mask), id(col_mask))
+        got = self._masked.get(key)
+        if got is not None and got[0] is row_mask and got[1] is col_mask:
+            return got[2]
+        if self.nnz == 0:
+            kill = np.zeros(0, dtype=np.intp)
+        else:
+            hit = (np.repeat(row_mask, np.diff(self._indptr64)) if row_mask is not None
+                   else np.zeros(self.nnz, dtype=bool))
+            if col_mask is not None:
+                hit |= col_mask[self.indices]
+            kill = np.flatnonzero(hit).astype(np.intp)
+        self._masked[key] = (row_mask, col_mask, kill)
+        return kill
+
+    # ------------------------------------------------------------- device plans
+    def device_plan(self, row_mask=None, col_mask=None, diag=None):
+        """The device scatter plan with the elimination folded in: rows in ``row_mask``
+        come out zero (with ``diag`` on their diagonal when it is given, which needs a
+        square pattern), entries in columns of ``col_mask`` are dropped.  Cached per
+        (masks, diagonal)."""
+        key = (id(row_mask), id(col_mask), diag)
+        got = self._plans.get(key)
+        if got is not None and got["row_mask"] is row_mask and got["col_mask"] is col_mask:
+            return got
+        dev = _lib.require_device()
+        base = self._plans.get("base")
+        if base is None:
+            base = self._plans["base"] = dict(
+                indptr=torch.as_tensor(self.indptr).to(dev),
+                rptr=torch.as_tensor(self.rptr.astype(np.int32)).to(dev),
+                runs=torch.as_tensor(self.runs).to(dev))
+        wide = self.max_row > 254
+        code_t = np.uint16 if wide else np.uint8
+        drop = np.iinfo(code_t).max
+        codes_key = ("codes", id(col_mask))
+        codes = self._plans.get(codes_key)
+        if codes is None or codes[0] is not col_mask:
+            c = self.pos.astype(code_t)
+            if col_mask is not None:
+                cm = np.asarray(col_mask, dtype=bool)[self.cdofs]            # (ne, nd_c)
+                dropped = np.broadcast_to(cm[:, None, :], (self.ne, self.nd_t, self.nd_c))
+                c = c.reshape(self.ne, self.nd_t, self.nd_c)
+                c[dropped] = drop
+                c = c.reshape(-1)
+            ct = torch.from_numpy(c.view(np.uint8) if not wide else c.view(np.int16)).to(dev)
+            codes = self._plans[codes_key] = (col_mask, ct)
+        rowmode = None
+        if row_mask is not None:
+            rm = np.asarray(row_mask, dtype=bool)
+            if diag is not None:
+                if not self.diag_first:
+                    raise ValueError("a diagonal is written on square patterns only")
+                rows = np.flatnonzero(rm)
+                first = self.indptr[rows]
+                if rows.size and not np.array_equal(self.indices[first], rows):
+                    raise RuntimeError("%d of %d eliminated rows have no diagonal entry"
+                                       % (int(np.count_nonzero(self.indices[first] != rows)),
+                                          rows.size))
+            rowmode = torch.from_numpy((rm.astype(np.int8) * (2 if diag is not None else 1))).to(dev)
+        plan = _lib.HmScatterPlan(
+            nrow=self.nrow, nnz=self.nnz, ne=self.ne, nd_t=self.nd_t, nd_c=self.nd_c,
+            code_bytes=2 if wide else 1, max_row=self.max_row,
+            indptr=_lib.ptr(base["indptr"]), rptr=_lib.ptr(base["rptr"]),
+            runs=_lib.ptr(base["runs"]), codes=_lib.ptr(codes[1]),
+            rowmode=_lib.ptr(rowmode), diag=0.0 if diag is None else float(diag))
+        got = dict(plan=plan, keep=(base, codes, rowmode), row_mask=row_mask, col_mask=col_mask)
+        self._plans[key] = got
+        return got
+
+    # ------------------------------------------------------------------ values
+    def data(self, element_matrices, zero_slots=None, host=False, dtype=None, plan=None,
+             out=None, stream=None):
+        """CSR ``data`` for one batch of element matrices (``ScatterPattern.data``,
+        ``pattern.py:749-791``): the entries of every slot summed in ascending entry order
+        on the device, then ``zero_slots`` cleared.  ``plan`` is a folded plan of
+        :meth:`device_plan` (then ``zero_slots`` is not needed).  ``dtype``: the values'
+        precision (that of the element matrices by default)."""
+        if len(element_matrices) != len(self.shapes):
+            raise ValueError("expected %d element arrays, got %d"
+                             % (len(self.shapes), len(element_matrices)))
+        E = _as_device(element_matrices[0])
+        if E.numel() != self.nent:
+            raise ValueError("element arrays hold %d entries, pattern expects %d"
+                             % (E.numel(), self.nent))
+        dtype = E.dtype if dtype is None else dtype
+        if out is None:
+            out = torch.empty(self.nnz, dtype=dtype, device=E.device)
+        p = plan if plan is not None else self.device_plan()
+        _lib.check(_lib.lib().hm_scatter_matrix(
+            _lib.ctypes.byref(p["plan"]), _lib.ptr(E), _dtype_code(E.dtype), 0, self.ne,
+            _lib.ptr(out), _dtype_code(out.dtype), 0, _lib.stream_handle(stream)), "scatter")
+        if zero_slots is not None and len(zero_slots):
+            out.index_fill_(0, _index_tensor(zero_slots, out.device), 0.0)
+        return out.cpu().numpy() if host else out
+
+    def fused_begin(self, dtype=torch.float64):
+        """A zeroed accumulator and the maps a chunked scatter feeds
+        (``fused_begin``, ``pattern.py:843-862``)."""
+        dev = _lib.require_device()
+        return torch.zeros(self.nnz, dtype=dtype, device=dev), {}
+
+    def fused_add(self, acc, maps, g, a, bnd, arr, plan=None, stream=None):
+        """Add the element matrices of elements ``[a, bnd)`` of group ``g``, in order
+        (``fused_add``, ``pattern.py:864-886``)."""
+        E = _as_device(arr)
+        if E.numel() != (bnd - a) * self.nd_t * self.nd_c:
+            raise ValueError("chunk [%d, %d) of group %d holds %d entries, pattern expects %d"
+                             % (a, bnd, g, E.numel(), (bnd - a) * self.nd_t * self.nd_c))
+        p = plan if plan is not None else self.device_plan()
+        _lib.check(_lib.lib().hm_scatter_matrix(
+            _lib.ctypes.byref(p["plan"]), _lib.ptr(E), _dtype_code(E.dtype), int(a), int(bnd),
+            _lib.ptr(acc), _dtype_code(acc.dtype), 1, _lib.stream_handle(stream)), "fused scatter")
+        return acc
+
+    def fused_end(self, acc, zero_slots=None, host=False):
+        if zero_slots is not None and len(zero_slots):
+            acc.index_fill_(0, _index_tensor(zero_slots, acc.device), 0.0)
+        return acc.cpu().numpy() if host else acc
+
+    def data_fused(self, chunks, zero_slots=None, host=False, dtype=torch.float64, plan=None):
+        acc, maps = self.fused_begin(dtype)
+        for g, a, bnd, arr in chunks:
+            acc = self.fused_add(acc, maps, g, a, bnd, arr, plan=plan)
+        return self.fused_end(acc, zero_slots, host=host)
+
+    def __repr__(self):
+        return "ScatterPattern(%dx%d, nnz=%d, entries=%d)" % (self.nrow, self.ncol, self.nnz,
+                                                            self.nent)
+
+
+class VectorPattern:
+    """The dof of every element-vector entry, grouped by dof (``VectorPattern``,
+    ``pattern.py:1156-1233``)."""
+
+    def __init__(self, tables, space=None):
+        if len(tables) != 1:
+            raise NotImplementedError("one element group (one element geometry) is supported")
+        t = tables[0]
+        self.n = int((space if space is not None else t.space).ndof)
+        self.ne, self.nd = int(t.group.ne), int(t.nd_total)
+        self.edofs = np.ascontiguousarray(t.edofs, dtype=np.int32)
+        zero = np.zeros((self.ne, 1), np.int32)
+        _, rptr, runs, _, _, _ = build_tables(self.edofs, zero, self.n, 1, False)
+        self.rptr, self.runs = rptr, runs
+        self._dev = None
+
+    def device(self):
+        if self._dev is None:
+            dev = _lib.require_device()
+            self._dev = (torch.as_tensor(self.rptr.astype(np.int32)).to(dev),
+                         torch.as_tensor(self.runs).to(dev))
+        return self._dev
+
+    def scatter(self, element_vectors, host=False, zero_rows=None, out=None, accumulate=False,
+                elements=None, stream=None):
+        """The local dof array of summed element vectors, in ascending entry order
+        (``VectorPattern.scatter``, ``pattern.py:1186-1209``); ``zero_rows`` (int8, one a
+        dof) clears the essential entries (``zero_essential``, ``bcs.py:80-102``)."""
+        V = _as_device(element_vectors[0])
+        a, bnd = (0, self.ne) if elements is None else elements
+        if V.numel() != (bnd - a) * self.nd:
+            raise ValueError("element vectors hold %d entries, pattern expects %d"
+                             % (V.numel(), (bnd - a) * self.nd))
+        if V.dtype != torch.float64:
+            V = V.double()
+        rptr, runs = self.device()
+        if out is None:
+            out = torch.zeros(self.n, dtype=torch.float64, device=V.device)
+        _lib.check(_lib.lib().hm_scatter_vector(
+            self.n, self.nd, _lib.ptr(rptr), _lib.ptr(runs), _lib.ptr(V), int(a), int(bnd),
+            _lib.ptr(zero_rows), _lib.ptr(out), int(bool(accumulate)),
+            _lib.stream_handle(stream)), "vector scatter")
+        return out.cpu().numpy() if host else out
+
+
+def _as_device(a):
+    if isinstance(a, torch.Tensor):
+        if not a.is_cuda:
+            a = a.to(_lib.require_device())
+        return a.contiguous()
+    return torch.as_tensor(np.ascontiguousarray(a)).to(_lib.require_device())
+
+
+def _dtype_code(dt):
+    if dt == torch.float64:
+        return _lib.FP64
+    if dt == torch.float32:
+        return _lib.FP32
+    raise ValueError("element arrays must be float64 or float32, not %s" % dt)
+
+
+def _index_tensor(idx, dev):
+    if isinstance(idx, torch.Tensor):
+        return idx.to(dev, torch.int64)
+    return torch.as_tensor(np.asarray(idx, dtype=np.int64)).to(dev)
+
+
+_PATTERNS = {}
+_VECTORS = {}
+
+
+def get_pattern(test_space, trial_space, groups):
+    """Cached :class:`ScatterPattern` per (test space, trial space, groups)
+    (``get_pattern``, ``pattern.py:1094-1127``)."""
+    key = (id(test_space), id(trial_space)) + tuple(id(g) for g in groups)
+    got = _PATTERNS.get(key)
+    if got is not None and got[0] is test_space and got[1] is trial_space:
+        return got[2]
+    pat = ScatterPattern([group_tables(test_space, g) for g in groups],
+                         [group_tables(trial_space, g) for g in groups],
+                         test_space=test_space, trial_space=trial_space)
+    _PATTERNS[key] = (test_space, trial_space, pat, tuple(groups))
+    return pat
+
+
+def get_vector_pattern(space, groups):
+    key = (id(space),) + tuple(id(g) for g in groups)
+    got = _VECTORS.get(key)
+    if got is not None and got[0] is space:
+        return got[1]
+    pat = VectorPattern([group_tables(space, g) for g in groups], space=space)
+    _VECTORS[key] = (space, pat, tuple(groups))
+    return pat
+
+
+def clear_pattern_cache():
+    _PATTERNS.clear()
+    _VECTORS.clear()

@@ -1,0 +1,146 @@
+"""The caller of the assembly path: ``PDEVariationalProblem``'s assembly methods
+(``hippymfem/modeling/PDEVariationalProblem.py``).
+
+It keeps, per block, the plan (pattern, folded elimination) and device buffers for the
+element matrices and the CSR values, so a repeated assembly (a Newton step, a serving
+request, a benchmark step) launches kernels into resident memory and allocates nothing.
+"""
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import ADJOINT, PARAMETER, STATE
+from .csrassemble import _device_plan, assemble_vector_csr, finish_block, plan_block
+from .kernel import QuadratureKernel
+
+NVAR = 3
+
+#: elimination per block (``PDEVariationalProblem.py:1448-1451``, ``:597-598``); a block
+#: with the parameter on a side keeps its columns (``_symmetric``, ``:118-124``)
+ELIMINATION = {
+    (ADJOINT, STATE): dict(test=True, diag_policy="one"),
+    (STATE, ADJOINT): dict(test=True, diag_policy="one"),
+    (STATE, STATE): dict(test=True, diag_policy="zero"),
+    (ADJOINT, PARAMETER): dict(test=True, diag_policy="one"),
+    (STATE, PARAMETER): dict(test=True, diag_policy="one"),
+    (PARAMETER, PARAMETER): dict(test=False, diag_policy="one"),
+}
+
+
+def _symmetric(i, j):
+    """Whether block (i, j) is eliminated on both sides where they share a space
+    (``PDEVariationalProblem._symmetric``)."""
+    return PARAMETER not in (i, j)
+
+
+class BlockAssembly:
+    """Resident buffers and plan of one block: ``assemble`` is one element-kernel launch
+    into ``E`` and one scatter launch into ``data``, both on one stream."""
+
+    def __init__(self, kernel, i, j, test_ess=None, diag_policy="one", precision="fp64",
+                 value_dtype=None):
+        self.kernel, self.i, self.j = kernel, i, j
+        Vi, Vj = kernel.spaces[i], kernel.spaces[j]
+        self.plan = plan_block(Vi, Vj, kernel.batches.groups, test_ess=test_ess,
+                               diag_policy=diag_policy, symmetric=_symmetric(i, j))
+        self.pattern = self.plan.pattern
+        self.dplan = _device_plan(self.plan)
+        self.precision = precision
+        dev = _lib.require_device()
+        e_dtype = torch.float64 if precision == "fp64" else torch.float32
+        if value_dtype is None:
+            value_dtype = e_dtype
+        ne = kernel.group.ne
+        self.E = torch.empty((ne, kernel.nd[i], kernel.nd[j]), dtype=e_dtype, device=dev)
+        self.data = torch.empty(self.pattern.nnz, dtype=value_dtype, device=dev)
+
+    def element_matrices(self, local_arrays, stream=None):
+        return self.kernel.element_matrices(self.i, self.j, local_arrays, precision=self.precision,
+                                            out=self.E, stream=stream)[0]
+
+    def scatter(self, stream=None):
+        return self.pattern.data([self.E], plan=self.dplan, out=self.data, stream=stream)
+
+    def assemble(self, local_arrays, stream=None):
+        self.element_matrices(local_arrays, stream)
+        self.scatter(stream)
+        return finish_block(self.plan, self.data)
+
+
+class PDEVariationalProblem:
+    """Assembly methods of the reference's forward problem for a density on ``Vh =
+    [Vu, Vm, Vu]`` with essential conditions ``bc0`` (``PDEVariationalProblem``,
+    ``PDEVariationalProblem.py:127-197``)."""
+
+    def __init__(self, Vh, density, batches, bc0, is_fwd_linear=None):
+        self.Vh = list(Vh)
+        self.density = density
+        self.batches = batches
+        self.bc0 = bc0
+        self.is_fwd_linear = density.is_fwd_linear if is_fwd_linear is None else is_fwd_linear
+        self.kernel = QuadratureKernel(density, self.Vh, batches)
+        self.nelem = batches.groups[0].ne
+        self._blocks = {}
+
+    def _locals(self, x):
+        return [x[STATE], x[PARAMETER], x[ADJOINT] if len(x) > ADJOINT and x[ADJOINT] is not None
+                else np.zeros(self.Vh[ADJOINT].ndof)]
+
+    def block_assembly(self, i, j, precision="fp64", value_dtype=None):
+        key = (i, j, precision, value_dtype)
+        got = self._blocks.get(key)
+        if got is None:
+            rule = ELIMINATION[(i, j)]
+            got = self._blocks[key] = BlockAssembly(
+                self.kernel, i, j, test_ess=self.bc0.ess_tdof if rule["test"] else None,
+                diag_policy=rule["diag_policy"], precision=precision, value_dtype=value_dtype)
+        return got
+
+    # ------------------------------------------------------------------ assembly
+    def residual(self, x, var=ADJOINT, ess=None):
+        """``dR/d(var)`` at ``x`` (``_residual``, ``PDEVariationalProblem.py:423-446``)."""
+        vecs = self.kernel.element_vectors(var, self._locals(x))
+        return assemble_vector_csr(self.Vh[var], self.batches.groups, vecs, self.nelem, ess=ess)
+
+    def block(self, i, j, x, precision="fp64"):
+        """Block ``(i, j)`` at ``x`` with its elimination (``_block``/``_domain_block``,
+        ``PDEVariationalProblem.py:448-525``)."""
+        return self.block_assembly(i, j, precision).assemble(self._locals(x))
+
+    def jacobian(self, x, precision="fp64"):
+        """The forward Jacobian ``A`` (``_jacobian``, ``PDEVariationalProblem.py:542-659``)."""
+        return self.block(ADJOINT, STATE, x, precision)
+
+    def set_linearization_point(self, x, gauss_newton_approx=False, precision="fp64"):
+        """``A`` and the Hessian blocks C, W_uu (nonlinear state), W_um, W_mm
+        (``setLinearizationPoint``, ``PDEVariationalProblem.py:1372-1482``)."""
+        need = [(ADJOINT, PARAMETER)]
+        if not gauss_newton_approx:
+            if not self.is_fwd_linear:
+                need.append((STATE, STATE))
+            need += [(STATE, PARAMETER), (PARAMETER, PARAMETER)]
+        out = {(ADJOINT, STATE): self.jacobian(x, precision)}
+        for ij in need:
+            out[ij] = self.block(ij[0], ij[1], x, precision)
+        return out
+
+    def apply_ij_at(self, i, j, x, dir, out=None):
+        """Block ``(i, j)`` at ``x`` applied to ``dir`` without assembling it
+        (``apply_ij_at``, ``PDEVariationalProblem.py:1672-1718``)."""
+        ess = self.bc0.ess_tdof
+        dvec = np.array(dir, dtype=np.float64, copy=True) if not isinstance(dir, torch.Tensor) \
+            else dir.detach().double().clone()
+        if j in (STATE, ADJOINT) and len(ess):
+            dvec[ess] = 0.0
+        vecs = self.kernel.element_hvp(i, j, self._locals(x), dvec)
+        res = assemble_vector_csr(self.Vh[i], self.batches.groups, vecs, self.nelem,
+                                  ess=ess if i in (STATE, ADJOINT) else None)
+        if i in (STATE, ADJOINT) and {i, j} == {STATE, ADJOINT} and len(ess):
+            src = dir if isinstance(dir, torch.Tensor) else torch.as_tensor(np.asarray(dir))
+            idx = torch.as_tensor(ess).to(res.device)
+            res[idx] = src.to(res.device, torch.float64)[idx]
+        if out is not None:
+            out.copy_(res)
+            return out
+        return res

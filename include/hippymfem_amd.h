@@ -1,0 +1,161 @@
+/* hippymfem_amd: C ABI of the MI355X (gfx950) finite-element assembly path.
+ *
+ * Plain pointers and sizes only.  Every device pointer is owned by the caller (torch
+ * tensors in the Python host layer); the library allocates nothing on the device.  Each
+ * device call is ordered on the stream it is given and returns 0 on success or a negative
+ * error code, whose text hm_last_error() gives.  Host calls (the pattern build) are
+ * synchronous and use the host's threads.
+ *
+ * Conventions (those of the reference):
+ *   slots STATE = 0 (u, H1 order p), PARAMETER = 1 (m, H1 order 1), ADJOINT = 2 (p);
+ *   block (i, j) = d2R / d(slot i dofs) d(slot j dofs): rows test slot i, columns trial
+ *   slot j (hippymfem/fem/kernel.py:1707-1709);
+ *   element-matrix entry k = (e * nd_i + a) * nd_j + b (hippymfem/fem/_patternkernels.py:103-105);
+ *   CSR rows of a square block lead with their diagonal, the other columns ascend
+ *   (hippymfem/fem/pattern.py:583-587), int32 indptr/indices as hypre's diag block.
+ */
+#ifndef HIPPYMFEM_AMD_H
+#define HIPPYMFEM_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { HM_STATE = 0, HM_PARAMETER = 1, HM_ADJOINT = 2 };
+enum { HM_DENSITY_LINEAR = 0, HM_DENSITY_NONLINEAR = 1 };
+/* arithmetic of an element contraction: FP64; FP32; BF16 or FP16 products with FP32
+ * accumulation and per-element scaling.  Geometry (J, det J, J^-1) is FP64 in all. */
+enum { HM_FP64 = 0, HM_FP32 = 1, HM_BF16 = 2, HM_FP16 = 3 };
+
+/* Errors. */
+enum {
+  HM_OK = 0,
+  HM_ERR_ARG = -1,       /* a bad argument (ValueError on the Python side)      */
+  HM_ERR_HIP = -2,       /* a HIP runtime error                                 */
+  HM_ERR_ROW_LONG = -3,  /* a row holds 2**16 entries or more                   */
+  HM_ERR_DUP = -4,       /* an element row repeats a column                     */
+  HM_ERR_NODIAG = -5,    /* an eliminated row has no diagonal entry            */
+  HM_ERR_UNSUPPORTED = -6
+};
+
+/* The text of the last error of this thread. */
+const char* hm_last_error(void);
+/* Library version, for the loader's check. */
+int hm_version(void);
+
+/* ------------------------------------------------------------------ the problem
+ * Everything the element kernels read, all device pointers.
+ *   verts      (nvert, 3) fp64       vertex coordinates
+ *   elem_verts (ne, 8)   int32       element vertices, local v = vx + 2 vy + 4 vz (trilinear map)
+ *   edofs_u    (ne, nd_u) int32      state/adjoint dofs of each element, in the table order
+ *   edofs_m    (ne, nd_m) int32      parameter dofs of each element
+ *   tab_u      N_u (nq, nd_u) then G_u (nq, nd_u, 3)  fp64 reference basis of the state space
+ *   tab_m      N_m (nq, nd_m) then G_m (nq, nd_m, 3)  fp64 reference basis of the parameter space
+ *   tab_x      G_x (nq, 8, 3)  fp64 gradients of the trilinear vertex basis
+ *   wq         (nq,) fp64 quadrature weights
+ *   u, m, p    dof vectors of the three slots (fp64)
+ * Replaces the element data of hippymfem/fem/elementbatch.py:159-223 (ElementGroup) and
+ * :413-507 (SpaceTables), which the reference keeps per element group. */
+typedef struct hm_problem {
+  int64_t ne;
+  int32_t nq, nd_u, nd_m, density;
+  double f;
+  const double* verts;
+  const int32_t* elem_verts;
+  const int32_t* edofs_u;
+  const int32_t* edofs_m;
+  const double* tab_u;
+  const double* tab_m;
+  const double* tab_x;
+  const double* wq;
+  const double* u;
+  const double* m;
+  const double* p;
+} hm_problem;
+
+/* Element matrices of block (i, j) for elements [e0, e1): out (e1-e0, nd_i, nd_j), fp64
+ * for HM_FP64, fp32 for the other precisions.  Replaces GroupKernel.hess_block /
+ * _hess_slot_quadrature_fn (hippymfem/fem/kernel.py:1704-1720, :1431-1485). */
+int hm_element_matrices(const hm_problem* P, int i, int j, int64_t e0, int64_t e1,
+                        int precision, void* out, void* stream);
+
+/* Element vectors dR_e/d(slot i) for [e0, e1): out (e1-e0, nd_i) fp64.
+ * Replaces GroupKernel.grad_block (hippymfem/fem/kernel.py:1343-1355). */
+int hm_element_vectors(const hm_problem* P, int i, int64_t e0, int64_t e1, double* out,
+                       void* stream);
+
+/* Block (i, j) applied element by element to the slot-j dof vector dir:
+ * out (e1-e0, nd_i) fp64.  Replaces GroupKernel.hvp_block (hippymfem/fem/kernel.py:1756-1782). */
+int hm_element_hvp(const hm_problem* P, int i, int j, const double* dir, int64_t e0,
+                   int64_t e1, double* out, void* stream);
+
+/* ---------------------------------------------------------------- the pattern (host)
+ * The sort-free pattern build of hippymfem/fem/patternbuild.py:209-253 (build_tables) and
+ * hippymfem/fem/_patternkernels.py:110-183, on the host's threads, in two calls.
+ *
+ * hm_pattern_build: tdofs (ne, nd_t), cdofs (ne, nd_c) int32 host arrays.  Writes
+ *   rptr (nrow+1) int64 and runs (ne*nd_t) int32: the element rows t = e*nd_t + a of
+ *   every CSR row, ascending (group_table); pos (ne*nd_t*nd_c) uint16: each entry's place
+ *   in its row (order_table); indptr (nrow+1) int64; *nnz; *max_row (longest row).
+ *   diag_first: rows lead with their diagonal (a square pattern).  nthreads <= 0: all.
+ * hm_pattern_place: indices (nnz) int32 and, if slot is not NULL, slot (ne*nd_t*nd_c)
+ *   int32 = indptr[row] + pos (place_table). */
+int hm_pattern_build(const int32_t* tdofs, const int32_t* cdofs, int64_t ne, int nd_t,
+                     int nd_c, int64_t nrow, int64_t ncol, int diag_first, int nthreads,
+                     int64_t* rptr, int32_t* runs, uint16_t* pos, int64_t* indptr,
+                     int64_t* nnz, int64_t* max_row);
+int hm_pattern_place(const int32_t* cdofs, int64_t ne, int nd_t, int nd_c, int64_t nrow,
+                     const int64_t* rptr, const int32_t* runs, const int64_t* indptr,
+                     const uint16_t* pos, int nthreads, int32_t* indices, int32_t* slot);
+/* *repeats = 1 when an element row puts two entries in one place (a repeated column),
+ * which the device scatter does not take (it adds one element row's entries at once). */
+int hm_pattern_repeats(int64_t ne, int nd_t, int nd_c, const uint16_t* pos, int nthreads,
+                       int32_t* repeats);
+
+/* ------------------------------------------------------------------ the scatter
+ * Device plan of one block's scatter: every CSR row sums the entries of its element rows
+ * in ascending entry order, which is np.bincount's order (hippymfem/fem/pattern.py:788),
+ * with no atomics.  Folded elimination: rowmode[r] = 0 assembles the row, 1 writes a zero
+ * row, 2 a zero row with `diag` at its first (diagonal) place; an entry whose code is
+ * all ones (0xFF / 0xFFFF) is dropped (an eliminated column).
+ *   indptr (nrow+1) int32, rptr (nrow+1) int32, runs (ne*nd_t) int32,
+ *   codes (ne*nd_t*nd_c) uint8 (code_bytes 1) or uint16 (2), rowmode (nrow) int8 or NULL. */
+typedef struct hm_scatter_plan {
+  int64_t nrow, nnz, ne;
+  int32_t nd_t, nd_c, code_bytes, max_row;
+  const int32_t* indptr;
+  const int32_t* rptr;
+  const int32_t* runs;
+  const void* codes;
+  const int8_t* rowmode;
+  double diag;
+} hm_scatter_plan;
+
+/* data[nnz] (+)= the entries of the element matrices of elements [e0, e1) (E holds
+ * exactly those, (e1-e0, nd_t, nd_c)).  e_dtype / out_dtype: HM_FP64 or HM_FP32.
+ * accumulate = 0 overwrites data (a whole batch), 1 adds to it (a chunk of a fused
+ * scatter, in order).  Replaces ScatterPattern.data / fused_add (hippymfem/fem/pattern.py:
+ * 749-791, :864-886) and the folded elimination (pattern.py:693-713; elimination.py:53-72). */
+int hm_scatter_matrix(const hm_scatter_plan* S, const void* E, int e_dtype, int64_t e0,
+                      int64_t e1, void* data, int out_dtype, int accumulate, void* stream);
+
+/* out[n] (+)= the element vectors V (e1-e0, nd) of [e0, e1) by dof, in entry order, then
+ * out[r] = 0 where zero_rows[r] != 0 (zero_rows may be NULL).  rptr/runs as the plan's.
+ * Replaces VectorPattern.scatter (hippymfem/fem/pattern.py:1186-1209) and zero_essential
+ * (hippymfem/fem/bcs.py:80-102). */
+int hm_scatter_vector(int64_t n, int nd, const int32_t* rptr, const int32_t* runs,
+                      const double* V, int64_t e0, int64_t e1, const int8_t* zero_rows,
+                      double* out, int accumulate, void* stream);
+
+/* y = A x for a CSR block (int32 indptr/indices, fp64 or fp32 values by val_dtype, fp64
+ * vectors).  Replaces HypreParMatrix::Mult on one rank (PDEVariationalProblem.apply_ij,
+ * hippymfem/modeling/PDEVariationalProblem.py:1629-1670). */
+int hm_csr_matvec(int64_t nrow, const int32_t* indptr, const int32_t* indices,
+                  const void* data, int val_dtype, const double* x, double* y, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,219 @@
+"""Parity of the HIP path with the CPU oracle (the reference's algorithm restated).
+
+Bar (the reference's own, ``hippymfem/test/_device.py:19-49``): CSR structure bit-exact,
+values within 1e-13 of the block's largest entry; given identical element matrices the
+scatter is bit-exact with ``np.bincount``.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import fe_oracle as O
+
+import hippymfem_amd as hm
+from hippymfem_amd import ADJOINT, PARAMETER, STATE
+from hippymfem_amd.assemble import assemble_matrix, assemble_vector
+from hippymfem_amd.workloads import Workload
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-13
+
+ELIM = {(ADJOINT, STATE): dict(test=True, diag_policy="one", symmetric=None),
+        (STATE, STATE): dict(test=True, diag_policy="zero", symmetric=None),
+        (ADJOINT, PARAMETER): dict(test=True, diag_policy="one", symmetric=False),
+        (STATE, PARAMETER): dict(test=True, diag_policy="one", symmetric=False),
+        (PARAMETER, PARAMETER): dict(test=False, diag_policy="one", symmetric=False)}
+
+
+def pair(n, **kw):
+    state = kw.pop("state", "uniform")
+    W = Workload(n, state=state, **kw)
+    P = O.Problem(n, order=kw.get("order", 2), perturb=kw.get("perturb", 0.0),
+                  numbering=kw.get("numbering", "lattice"), state=state,
+                  density="nonlinear" if kw.get("nonlinear") else "linear")
+    return W, P
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    scale = max(np.abs(b).max(), 1e-300)
+    return float(np.abs(a - b).max() / scale)
+
+
+def assemble_gpu(W, i, j, mats=None):
+    rule = ELIM[(i, j)]
+    mats = W.kernel.element_matrices(i, j, W.locals()) if mats is None else mats
+    return assemble_matrix(W.spaces[i], W.spaces[j], W.groups, mats, W.mesh.ne,
+                           test_ess=W.ess if rule["test"] else None,
+                           diag_policy=rule["diag_policy"], symmetric=rule["symmetric"])
+
+
+BLOCKS = [(ADJOINT, STATE), (ADJOINT, PARAMETER), (STATE, PARAMETER), (PARAMETER, PARAMETER)]
+
+
+@pytest.mark.parametrize("numbering", ["lattice", "mfem", "shuffled"])
+@pytest.mark.parametrize("nonlinear", [False, True])
+def test_element_matrices(gpu, numbering, nonlinear):
+    W, P = pair(3, numbering=numbering, nonlinear=nonlinear)
+    blocks = BLOCKS + ([(STATE, STATE)] if nonlinear else [])
+    for i, j in blocks:
+        E = W.kernel.element_matrices(i, j, W.locals())[0].cpu().numpy()
+        Eo = P.block(i, j)
+        assert E.shape == Eo.shape
+        assert rel(E, Eo) <= TOL, ((i, j), rel(E, Eo))
+
+
+@pytest.mark.parametrize("perturb", [0.0, 0.2])
+def test_element_matrices_perturbed(gpu, perturb):
+    W, P = pair(4, perturb=perturb, nonlinear=True, state="fourier")
+    for i, j in BLOCKS + [(STATE, STATE), (STATE, ADJOINT), (PARAMETER, STATE)]:
+        E = W.kernel.element_matrices(i, j, W.locals())[0].cpu().numpy()
+        assert rel(E, P.block(i, j)) <= TOL, (i, j)
+
+
+@pytest.mark.parametrize("nonlinear", [False, True])
+def test_element_vectors(gpu, nonlinear):
+    W, P = pair(4, perturb=0.2, nonlinear=nonlinear)
+    for i in (STATE, PARAMETER, ADJOINT):
+        V = W.kernel.element_vectors(i, W.locals())[0].cpu().numpy()
+        assert rel(V, P.vector(i)) <= TOL, i
+
+
+def test_element_hvp(gpu):
+    W, P = pair(4, perturb=0.2, nonlinear=True)
+    rng = np.random.default_rng(9)
+    for i, j in BLOCKS + [(STATE, STATE)]:
+        d = rng.standard_normal(W.spaces[j].ndof)
+        y = W.kernel.element_hvp(i, j, W.locals(), d)[0].cpu().numpy()
+        yo = O.element_hvp(P.data, P.density, i, j, P.u, P.m, P.p, d)
+        assert rel(y, yo) <= TOL, (i, j)
+
+
+def test_config0_jacobian_and_residual(gpu):
+    """16^3 P2, linear density: A and the residual against the oracle."""
+    W, P = pair(16)
+    A = assemble_gpu(W, ADJOINT, STATE)
+    ip, ix, d = A.host_csr()
+    ipo, ixo, do = P.assembled(ADJOINT, STATE)
+    assert A.nnz == 129 ** 3 == ipo[-1]
+    assert np.array_equal(ip, ipo) and ip.dtype == np.int32
+    assert np.array_equal(ix, ixo) and ix.dtype == np.int32
+    assert rel(d, do) <= TOL
+    r = assemble_vector(W.Vu, W.groups, W.kernel.element_vectors(ADJOINT, W.locals()),
+                        W.mesh.ne, ess=W.ess).cpu().numpy()
+    assert rel(r, P.residual(ADJOINT)) <= TOL
+
+
+@pytest.mark.parametrize("numbering", ["lattice", "mfem"])
+@pytest.mark.parametrize("nonlinear", [False, True])
+def test_all_blocks_assembled(gpu, numbering, nonlinear):
+    W, P = pair(4, numbering=numbering, nonlinear=nonlinear, perturb=0.2)
+    blocks = BLOCKS + ([(STATE, STATE)] if nonlinear else [])
+    for i, j in blocks:
+        A = assemble_gpu(W, i, j)
+        ip, ix, d = A.host_csr()
+        ipo, ixo, do = P.assembled(i, j)
+        assert np.array_equal(ip, ipo) and np.array_equal(ix, ixo), (i, j)
+        assert rel(d, do) <= TOL, (i, j)
+
+
+def test_scatter_bitexact_with_bincount(gpu):
+    """Given the oracle's element matrices, the device scatter equals np.bincount bit for
+    bit (ascending entry order per slot), eliminated slots and diagonal included."""
+    W, P = pair(6, perturb=0.2, nonlinear=True)
+    for i, j in BLOCKS + [(STATE, STATE)]:
+        Eo = P.block(i, j)
+        A = assemble_gpu(W, i, j, mats=[torch.as_tensor(Eo).to(gpu)])
+        _, _, d = A.host_csr()
+        _, _, do = P.assembled(i, j, E=Eo)
+        assert np.array_equal(d, do), (i, j, np.abs(d - do).max())
+
+
+def test_fused_chunks_equal_whole(gpu):
+    W, _ = pair(6)
+    whole = assemble_gpu(W, ADJOINT, STATE).data.cpu().numpy()
+    for chunk in (1, 7, 50, 216):
+        def chunks(chunk=chunk):
+            return W.kernel.element_matrix_chunks(ADJOINT, STATE, W.locals(), chunk=chunk)
+        A = assemble_matrix(W.Vu, W.Vu, W.groups, chunks, W.mesh.ne, test_ess=W.ess)
+        assert np.array_equal(A.data.cpu().numpy(), whole), chunk
+
+
+def test_repeat_bit_identical(gpu):
+    W, _ = pair(8)
+    ref = assemble_gpu(W, ADJOINT, STATE).data.cpu().numpy()
+    for _ in range(4):
+        assert np.array_equal(assemble_gpu(W, ADJOINT, STATE).data.cpu().numpy(), ref)
+
+
+def test_matvec_and_matrix_free(gpu):
+    W, P = pair(6)
+    A = assemble_gpu(W, ADJOINT, STATE)
+    x = W.direction()
+    y = A.mult(x).cpu().numpy()
+    ip, ix, d = P.assembled(ADJOINT, STATE)
+    assert rel(y, O.csr_matvec(ip, ix, d, x)) <= TOL
+
+
+def test_zero_slots_and_keep_policy(gpu):
+    W, P = pair(4)
+    E = W.kernel.element_matrices(ADJOINT, STATE, W.locals())
+    pat = hm.get_pattern(W.Vu, W.Vu, W.groups)
+    mask = np.zeros(W.Vu.ndof, bool)
+    mask[W.ess] = True
+    kill = pat.masked_slots(mask, mask)
+    d = pat.data(E, zero_slots=kill).cpu().numpy()
+    ip, ix, _ = P.assembled(ADJOINT, STATE)
+    Eo = P.block(ADJOINT, STATE)
+    slot = O.pattern_tables_sort(P.Vu.edofs, P.Vu.edofs, P.Vu.ndof, P.Vu.ndof, True)[2]
+    do = O.scatter(slot, Eo, len(ix), O.masked_slots(ip, ix, mask, mask))
+    assert rel(d, do) <= TOL
+    A = assemble_matrix(W.Vu, W.Vu, W.groups, E, W.mesh.ne, test_ess=W.ess,
+                        diag_policy="keep")
+    raw = O.scatter(slot, Eo, len(ix))
+    dk = A.data.cpu().numpy()
+    first = ip[W.ess]
+    assert rel(dk[first], raw[first]) <= TOL
+    others = np.setdiff1d(kill, first)
+    assert np.all(dk[others] == 0.0)
+
+
+def test_errors(gpu):
+    W, _ = pair(2)
+    pat = hm.get_pattern(W.Vu, W.Vu, W.groups)
+    with pytest.raises(ValueError):
+        pat.data([torch.zeros(3, device=gpu, dtype=torch.float64)])
+    with pytest.raises(ValueError):
+        W.kernel.element_matrices(ADJOINT, STATE, W.locals(), elements=(0, W.mesh.ne + 1))
+
+
+def test_full_size_properties(gpu):
+    """64^3 P2 (config 1): structure closed form, symmetry of A, agreement of the assembled
+    product with the matrix-free one, and the oracle on a sample of elements."""
+    W = Workload(64)
+    A = assemble_gpu(W, ADJOINT, STATE)
+    assert A.nnz == 513 ** 3
+    ip = A.pattern.indptr
+    assert ip[-1] == 513 ** 3 and ip.dtype == np.int32
+    x = W.direction()
+    y = A.mult(x)
+    xz = x.copy()
+    xz[W.ess] = 0.0
+    yf = assemble_vector(W.Vu, W.groups, W.kernel.element_hvp(ADJOINT, STATE, W.locals(), xz),
+                         W.mesh.ne, ess=W.ess)
+    yf[torch.as_tensor(W.ess).to(gpu)] = torch.as_tensor(x[W.ess]).to(gpu)
+    assert rel(y.cpu().numpy(), yf.cpu().numpy()) <= TOL
+    # symmetric: y . A z == z . A y
+    z = np.random.default_rng(5).standard_normal(W.Vu.ndof)
+    az = A.mult(z).cpu().numpy()
+    ay = y.cpu().numpy()
+    assert abs(ay @ z - az @ x) <= 1e-11 * abs(ay @ z)
+    # a sample of elements against the oracle
+    P = O.Problem(64)
+    elems = np.sort(np.random.default_rng(3).choice(W.mesh.ne, 512, replace=False))
+    E = W.kernel.element_matrices(ADJOINT, STATE, W.locals())[0]
+    Eg = E[torch.as_tensor(elems).to(gpu)].cpu().numpy()
+    assert rel(Eg, P.block(ADJOINT, STATE, elems)) <= TOL
