@@ -1,0 +1,141 @@
+"""CPU tests of the host side: the golden fixtures, the product's host logic against the
+oracle (pattern build, tables, fields, boundary dofs), and the C ABI's exports."""
+
+import ctypes
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import cref
+from oracle import fe_oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+CASES = {"hex2_p1_linear": (2, 1, 0.0, "linear", "lattice"),
+         "hex2_p2_nonlinear_perturbed": (2, 2, 0.2, "nonlinear", "lattice"),
+         "hex4_p2_linear": (4, 2, 0.0, "linear", "lattice"),
+         "hex3_p2_linear_mfem": (3, 2, 0.0, "linear", "mfem")}
+BLOCKS = {"A": (2, 0), "C": (2, 1), "Wum": (0, 1), "Wmm": (1, 1), "Wuu": (0, 0)}
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_golden_fixtures(name):
+    """The oracle still gives the committed fixtures (structure exactly, values to 1e-14)."""
+    g = np.load(os.path.join(GOLDEN, name + ".npz"))
+    n, order, perturb, density, numbering = CASES[name]
+    P = O.Problem(n, order=order, perturb=perturb, density=density, numbering=numbering)
+    assert np.array_equal(g["edofs_u"], P.Vu.edofs) and np.array_equal(g["ess"], P.ess)
+    assert np.array_equal(g["m"], P.m) and np.array_equal(g["u"], P.u)
+    r = P.residual(2)
+    assert np.abs(r - g["residual"]).max() <= 1e-14 * np.abs(g["residual"]).max()
+    for key, (i, j) in BLOCKS.items():
+        if key + "_data" not in g:
+            continue
+        ip, ix, d = P.assembled(i, j)
+        assert np.array_equal(ip, g[key + "_indptr"]) and np.array_equal(ix, g[key + "_indices"])
+        assert np.abs(d - g[key + "_data"]).max() <= 1e-14 * np.abs(d).max(), key
+
+
+# ------------------------------------------------------------- product host logic
+hm_lib = pytest.importorskip("hippymfem_amd._lib")
+
+
+def _built():
+    return os.path.exists(hm_lib.LIB_PATH)
+
+
+needs_lib = pytest.mark.skipif(not _built(), reason="native library not built")
+
+
+@needs_lib
+def test_library_exports_every_declared_symbol():
+    header = open(os.path.join(ROOT, "include", "hippymfem_amd.h")).read()
+    declared = set(re.findall(r"^\s*(?:int|const char\*)\s+(hm_\w+)\s*\(", header, re.M))
+    assert declared == set(hm_lib.EXPORTS)
+    L = ctypes.CDLL(hm_lib.LIB_PATH)
+    for name in declared:
+        assert hasattr(L, name), name
+    assert hm_lib.lib().hm_version() >= 1
+
+
+@needs_lib
+@pytest.mark.parametrize("numbering", ["lattice", "mfem", "shuffled"])
+def test_product_host_matches_oracle(numbering):
+    from hippymfem_amd.elementbatch import group_tables
+    from hippymfem_amd.pattern import ScatterPattern, VectorPattern
+    from hippymfem_amd.workloads import Workload
+
+    W = Workload(3, numbering=numbering, perturb=0.2)
+    P = O.Problem(3, numbering=numbering, perturb=0.2)
+    assert np.array_equal(W.mesh.verts, P.mesh.verts)
+    assert np.array_equal(W.Vu.edofs, P.Vu.edofs) and np.array_equal(W.Vm.edofs, P.Vm.edofs)
+    assert np.array_equal(W.m, P.m) and np.array_equal(W.u, P.u) and np.array_equal(W.p, P.p)
+    assert np.array_equal(W.ess, P.ess)
+    g = W.groups[0]
+    tu = group_tables(W.Vu, g)
+    assert np.abs(tu.N - P.data.Nu).max() < 1e-15 and np.abs(tu.G - P.data.Gu).max() < 1e-14
+    assert np.array_equal(g.w, P.data.w)
+    for (T, C) in ((W.Vu, W.Vu), (W.Vu, W.Vm), (W.Vm, W.Vu), (W.Vm, W.Vm)):
+        S = ScatterPattern([group_tables(T, g)], [group_tables(C, g)], T, C)
+        ip, ix, slot, nnz = cref.pattern_tables(T.edofs, C.edofs, T.ndof, C.ndof, T is C)
+        assert S.nnz == nnz
+        assert S.indptr.dtype == np.int32 and np.array_equal(S.indptr, ip)
+        assert S.indices.dtype == np.int32 and np.array_equal(S.indices, ix)
+        assert np.array_equal(S.slot, slot)
+        # runs: element rows per CSR row, ascending
+        rc, runs = O.runs_of_rows(T.edofs, T.ndof)
+        assert np.array_equal(S.rptr, rc) and np.array_equal(S.runs, runs)
+    V = VectorPattern([tu], W.Vu)
+    rc, runs = O.runs_of_rows(W.Vu.edofs, W.Vu.ndof)
+    assert np.array_equal(V.rptr, rc) and np.array_equal(V.runs, runs)
+
+
+@needs_lib
+def test_product_pattern_errors():
+    from hippymfem_amd.pattern import build_tables
+
+    tdofs = np.zeros((2500, 27), np.int32)
+    cdofs = np.tile(np.arange(27, dtype=np.int32), (2500, 1))
+    with pytest.raises(ValueError):
+        build_tables(tdofs, cdofs, 1, 27, False)
+    with pytest.raises(ValueError):
+        build_tables(np.full((1, 2), 5, np.int32), np.zeros((1, 2), np.int32), 3, 3, False)
+
+
+@needs_lib
+def test_product_masked_slots_match_oracle():
+    from hippymfem_amd.elementbatch import group_tables
+    from hippymfem_amd.pattern import ScatterPattern
+    from hippymfem_amd.workloads import Workload
+
+    W = Workload(3)
+    g = W.groups[0]
+    S = ScatterPattern([group_tables(W.Vu, g)], [group_tables(W.Vu, g)], W.Vu, W.Vu)
+    mask = O.ess_mask(W.Vu.ndof, W.ess)
+    assert np.array_equal(S.masked_slots(mask, mask), O.masked_slots(S.indptr, S.indices, mask, mask))
+    assert np.array_equal(S.masked_slots(mask, None), O.masked_slots(S.indptr, S.indices, mask, None))
+
+
+def test_bench_refuses_several_gpus():
+    res = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                         capture_output=True, text=True, timeout=120)
+    assert res.returncode != 0 and "one GPU" in res.stderr
+
+
+def test_product_fails_loudly_without_gpu():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    if not _built():
+        pytest.skip("native library not built")
+    from hippymfem_amd.workloads import Workload
+
+    W = Workload(2)
+    with pytest.raises(RuntimeError):
+        W.kernel.element_matrices(2, 0, W.locals())
