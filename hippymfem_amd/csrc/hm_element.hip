@@ -10,131 +10,13 @@
 // with w = w_q |det J_q|.  The two are equal in exact arithmetic.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include "hm_common.h"
 #include "hm_device.h"
+#include "hm_point.h"
 
 namespace hm {
-
-// Everything one quadrature point of one element needs, in FP64.
-struct PointState {
-  double w;           // w_q |det J|
-  double Ji[3][3];    // Jinv[k][d] = dxi_k / dx_d
-  double kap, g, g1, g2;
-  double gu[3], gp[3];  // physical gradients of u and p
-  double dot;           // grad u . grad p
-  double uq;            // value of u
-};
-
-__device__ inline void point_state(const hm_problem& P, int q, const double* Xe,
-                                   const double* ue, const double* me, const double* pe,
-                                   PointState& S) {
-  const int nq = P.nq;
-  // J[d][k] = sum_v X_v[d] dN_v/dxi_k (trilinear map)
-  double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-  const double* gx = P.tab_x + (size_t)q * 24;
-  for (int v = 0; v < 8; ++v)
-    for (int d = 0; d < 3; ++d)
-      for (int k = 0; k < 3; ++k) J[d][k] = fma(Xe[v * 3 + d], gx[v * 3 + k], J[d][k]);
-  const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
-  const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
-  const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
-  const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
-  const double id = 1.0 / det;
-  // inverse of J (rows k = reference direction, columns d = physical)
-  S.Ji[0][0] = c00 * id;
-  S.Ji[1][0] = c01 * id;
-  S.Ji[2][0] = c02 * id;
-  S.Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
-  S.Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
-  S.Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
-  S.Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
-  S.Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
-  S.Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
-  S.w = P.wq[q] * fabs(det);
-  // fields: values and reference gradients, then physical gradients
-  const int nu = P.nd_u, nm = P.nd_m;
-  const double* Nu = P.tab_u + (size_t)q * nu;
-  const double* Gu = P.tab_u + (size_t)nq * nu + (size_t)q * nu * 3;
-  const double* Nm = P.tab_m + (size_t)q * nm;
-  double uq = 0, mq = 0, ru[3] = {0, 0, 0}, rp[3] = {0, 0, 0};
-  for (int a = 0; a < nu; ++a) {
-    uq = fma(Nu[a], ue[a], uq);
-    for (int k = 0; k < 3; ++k) {
-      ru[k] = fma(Gu[a * 3 + k], ue[a], ru[k]);
-      rp[k] = fma(Gu[a * 3 + k], pe[a], rp[k]);
-    }
-  }
-  for (int a = 0; a < nm; ++a) mq = fma(Nm[a], me[a], mq);
-  for (int d = 0; d < 3; ++d) {
-    S.gu[d] = S.Ji[0][d] * ru[0] + S.Ji[1][d] * ru[1] + S.Ji[2][d] * ru[2];
-    S.gp[d] = S.Ji[0][d] * rp[0] + S.Ji[1][d] * rp[1] + S.Ji[2][d] * rp[2];
-  }
-  S.dot = S.gu[0] * S.gp[0] + S.gu[1] * S.gp[1] + S.gu[2] * S.gp[2];
-  S.kap = exp(mq);
-  S.uq = uq;
-  if (P.density == HM_DENSITY_NONLINEAR) {
-    S.g = 1.0 + uq * uq;
-    S.g1 = 2.0 * uq;
-    S.g2 = 2.0;
-  } else {
-    S.g = 1.0;
-    S.g1 = 0.0;
-    S.g2 = 0.0;
-  }
-}
-
-// The pointwise block of test slot s and trial slot j, in the reference-gradient basis,
-// D[r][c] with r, c in {value, d/dxi_0, d/dxi_1, d/dxi_2}, times w.  The closed forms of
-// the density r = exp(m) g(u) grad u . grad p - f p (second derivatives per part).
-__device__ inline void point_block(int s, int j, const PointState& S, double D[4][4]) {
-  double vv = 0, vg[3] = {0, 0, 0}, gv[3] = {0, 0, 0}, gg = 0;
-  const double kg = S.kap * S.g, kg1 = S.kap * S.g1;
-  const int key = s * 3 + j;
-  switch (key) {
-    case HM_STATE * 3 + HM_STATE:
-      vv = S.kap * S.g2 * S.dot;
-      for (int d = 0; d < 3; ++d) vg[d] = gv[d] = kg1 * S.gp[d];
-      break;
-    case HM_STATE * 3 + HM_PARAMETER:
-      vv = kg1 * S.dot;
-      for (int d = 0; d < 3; ++d) gv[d] = kg * S.gp[d];
-      break;
-    case HM_PARAMETER * 3 + HM_STATE:
-      vv = kg1 * S.dot;
-      for (int d = 0; d < 3; ++d) vg[d] = kg * S.gp[d];
-      break;
-    case HM_STATE * 3 + HM_ADJOINT:
-      for (int d = 0; d < 3; ++d) vg[d] = kg1 * S.gu[d];
-      gg = kg;
-      break;
-    case HM_ADJOINT * 3 + HM_STATE:
-      for (int d = 0; d < 3; ++d) gv[d] = kg1 * S.gu[d];
-      gg = kg;
-      break;
-    case HM_PARAMETER * 3 + HM_PARAMETER:
-      vv = kg * S.dot;
-      break;
-    case HM_PARAMETER * 3 + HM_ADJOINT:
-      for (int d = 0; d < 3; ++d) vg[d] = kg * S.gu[d];
-      break;
-    case HM_ADJOINT * 3 + HM_PARAMETER:
-      for (int d = 0; d < 3; ++d) gv[d] = kg * S.gu[d];
-      break;
-    default:
-      break;
-  }
-  const double w = S.w;
-  D[0][0] = w * vv;
-  for (int k = 0; k < 3; ++k) {
-    D[0][1 + k] = w * (S.Ji[k][0] * vg[0] + S.Ji[k][1] * vg[1] + S.Ji[k][2] * vg[2]);
-    D[1 + k][0] = w * (S.Ji[k][0] * gv[0] + S.Ji[k][1] * gv[1] + S.Ji[k][2] * gv[2]);
-  }
-  for (int k = 0; k < 3; ++k)
-    for (int l = 0; l < 3; ++l) {
-      const double K = S.Ji[k][0] * S.Ji[l][0] + S.Ji[k][1] * S.Ji[l][1] + S.Ji[k][2] * S.Ji[l][2];
-      D[1 + k][1 + l] = w * gg * K;
-    }
-}
 
 // Loads the element's vertex coordinates and its three dof vectors into LDS.
 __device__ inline void load_element(const hm_problem& P, int64_t e, double* Xe, double* ue,
@@ -157,26 +39,6 @@ __device__ inline void load_basis(const double* tab, int nq, int nd, double* Psi
     const int a = i % nd, r = (i / nd) % 4, q = i / (4 * nd);
     Psi[i] = r == 0 ? tab[q * nd + a] : tab[(size_t)nq * nd + ((size_t)q * nd + a) * 3 + (r - 1)];
   }
-}
-
-// Which parts of the (s, j) block can be nonzero: bit 0 the value row, bit 1 the gradient
-// rows, bit 2 the value column, bit 3 the gradient columns (the reference skips the
-// identically zero pointwise blocks the same way, kernel.py:1606-1646).
-__host__ __device__ inline int block_parts(int s, int j, int density) {
-  const bool nl = density == HM_DENSITY_NONLINEAR;
-  bool vv = false, vg = false, gv = false, gg = false;
-  switch (s * 3 + j) {
-    case 0: vv = vg = gv = nl; break;                    // (u, u)
-    case 1: vv = nl; gv = true; break;                   // (u, m)
-    case 3: vv = nl; vg = true; break;                   // (m, u)
-    case 2: vg = nl; gg = true; break;                   // (u, p)
-    case 6: gv = nl; gg = true; break;                   // (p, u)
-    case 4: vv = true; break;                            // (m, m)
-    case 5: vg = true; break;                            // (m, p)
-    case 7: gv = true; break;                            // (p, m)
-    default: break;
-  }
-  return ((vv || vg) ? 1 : 0) | ((gv || gg) ? 2 : 0) | ((vv || gv) ? 4 : 0) | ((vg || gg) ? 8 : 0);
 }
 
 template <typename OutT>
@@ -356,7 +218,16 @@ extern "C" int hm_element_matrices(const hm_problem* P, int i, int j, int64_t e0
   HM_REQUIRE(e0 >= 0 && e0 <= e1 && e1 <= P->ne, "elements [%lld, %lld) outside [0, %lld)",
              (long long)e0, (long long)e1, (long long)P->ne);
   if (e1 == e0) return HM_OK;
-  if (precision != HM_FP64) return hm_element_matrices_lowp(P, i, j, e0, e1, precision, out, stream);
+  const int nd_ti = i == HM_PARAMETER ? P->nd_m : P->nd_u;
+  const int nd_cj = j == HM_PARAMETER ? P->nd_m : P->nd_u;
+  const char* route = getenv("HM_CONTRACTION");
+  const bool valu = route && route[0] == 'v';
+  if (precision == HM_BF16 || precision == HM_FP16)
+    return hm_element_matrices_lowp(P, i, j, e0, e1, precision, out, stream);
+  HM_REQUIRE(precision == HM_FP64 || precision == HM_FP32, "unknown precision %d", precision);
+  if (nd_ti <= 32 && nd_cj <= 32 && (!valu || precision == HM_FP32))
+    return hm_element_matrices_mfma(P, i, j, e0, e1, precision, out, stream);
+  HM_REQUIRE(precision == HM_FP64, "the FP32 contraction needs at most 32 dofs per element");
   const int nq = P->nq;
   const int nd_t = i == HM_PARAMETER ? P->nd_m : P->nd_u;
   const int nd_c = j == HM_PARAMETER ? P->nd_m : P->nd_u;

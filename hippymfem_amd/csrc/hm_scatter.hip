@@ -29,18 +29,23 @@ struct Drop<uint16_t> {
   static constexpr unsigned v = 0xFFFFu;
 };
 
-// LPR lanes reduce one row; a block of 256 threads holds 256/LPR row buffers.
+// LPR lanes reduce one row; a block of 256 threads holds 256/LPR row buffers.  The runs
+// of a row are taken eight at a time: their ids are read by one load and spread with a
+// lane broadcast, all eight runs' entries and codes are loaded before any is added (eight
+// loads in flight per lane), then added run after run in ascending order.
 template <typename CodeT, typename ET, typename OT, int LPR>
 __global__ void __launch_bounds__(256) k_scatter_rows(hm_scatter_plan S, const ET* __restrict__ E,
                                                       int64_t t0, int64_t t1, OT* __restrict__ data,
                                                       int accumulate) {
   extern __shared__ double rowbuf[];
   constexpr int groups = 256 / LPR;
+  constexpr int BATCH = 8;
   const int lane = threadIdx.x % LPR;
   const int grp = threadIdx.x / LPR;
   double* buf = rowbuf + (size_t)grp * S.max_row;
   const CodeT* codes = (const CodeT*)S.codes;
   const int nd_c = S.nd_c;
+  constexpr unsigned DROP = Drop<CodeT>::v;
   for (int64_t r = (int64_t)blockIdx.x * groups + grp; r < S.nrow; r += (int64_t)gridDim.x * groups) {
     const int32_t lo = S.indptr[r];
     const int32_t len = S.indptr[r + 1] - lo;
@@ -53,17 +58,30 @@ __global__ void __launch_bounds__(256) k_scatter_rows(hm_scatter_plan S, const E
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     const int32_t p1 = S.rptr[r + 1];
-    for (int32_t p = S.rptr[r]; p < p1; ++p) {
-      const int64_t t = S.runs[p];
-      if (t < t0 || t >= t1) continue;
-      const int64_t kc = t * nd_c;
-      const int64_t ke = (t - t0) * nd_c;
-      for (int b = lane; b < nd_c; b += LPR) {
-        const unsigned c = codes[kc + b];
-        if (c != Drop<CodeT>::v) buf[c] += (double)E[ke + b];
+    for (int32_t p = S.rptr[r]; p < p1; p += BATCH) {
+      const int m = p1 - p < BATCH ? p1 - p : BATCH;
+      const int32_t mine = lane < m ? S.runs[p + lane] : -1;
+      for (int b0 = 0; b0 < nd_c; b0 += LPR) {
+        const int b = b0 + lane;
+        double v[BATCH];
+        unsigned c[BATCH];
+#pragma unroll
+        for (int k = 0; k < BATCH; ++k) {
+          const int64_t t = __shfl(mine, k, LPR);
+          c[k] = DROP;
+          v[k] = 0.0;
+          if (k < m && b < nd_c && t >= t0 && t < t1) {
+            c[k] = __builtin_nontemporal_load(codes + t * nd_c + b);
+            v[k] = (double)__builtin_nontemporal_load(E + (t - t0) * nd_c + b);
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < BATCH; ++k) {
+          if (c[k] != DROP) buf[c[k]] += v[k];
+          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+        }
       }
-      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-      __builtin_amdgcn_wave_barrier();
     }
     for (int i = lane; i < len; i += LPR) data[lo + i] = (OT)buf[i];
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");

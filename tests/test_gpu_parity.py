@@ -217,3 +217,34 @@ def test_full_size_properties(gpu):
     E = W.kernel.element_matrices(ADJOINT, STATE, W.locals())[0]
     Eg = E[torch.as_tensor(elems).to(gpu)].cpu().numpy()
     assert rel(Eg, P.block(ADJOINT, STATE, elems)) <= TOL
+
+
+@pytest.mark.parametrize("nonlinear", [False, True])
+def test_mfma_route_equals_valu_route(gpu, nonlinear, monkeypatch):
+    """The matrix-core contraction and the plain FMA one agree to round-off."""
+    W, _ = pair(4, perturb=0.2, nonlinear=nonlinear)
+    blocks = BLOCKS + ([(STATE, STATE)] if nonlinear else [])
+    for i, j in blocks:
+        Em = W.kernel.element_matrices(i, j, W.locals())[0].cpu().numpy()
+        monkeypatch.setenv("HM_CONTRACTION", "valu")
+        Ev = W.kernel.element_matrices(i, j, W.locals())[0].cpu().numpy()
+        monkeypatch.delenv("HM_CONTRACTION")
+        assert rel(Em, Ev) <= TOL, (i, j)
+
+
+@pytest.mark.parametrize("nonlinear", [False, True])
+def test_fp32_contraction_error(gpu, nonlinear):
+    """The FP32 matrix-core form is within 1e-6 of the FP64 oracle, element matrices and
+    assembled blocks (values accumulated from FP32 entries)."""
+    W, P = pair(6, perturb=0.2, nonlinear=nonlinear)
+    blocks = BLOCKS + ([(STATE, STATE)] if nonlinear else [])
+    for i, j in blocks:
+        E = W.kernel.element_matrices(i, j, W.locals(), precision="fp32")[0]
+        assert E.dtype == torch.float32
+        assert rel(E.cpu().numpy(), P.block(i, j)) <= 1e-6, (i, j)
+        A = assemble_gpu(W, i, j, mats=[E])
+        assert A.data.dtype == torch.float32
+        ip, ix, d = A.host_csr()
+        ipo, ixo, do = P.assembled(i, j)
+        assert np.array_equal(ip, ipo) and np.array_equal(ix, ixo)
+        assert rel(d, do) <= 1e-6, (i, j, rel(d, do))
