@@ -78,6 +78,38 @@ def cpu_baseline(n):
                       "%.1f us/elem" % (n, P.mesh.ne, nnz, reps, 1e6 * dt / P.mesh.ne)}
 
 
+def fp32_form(prob, loc, blk64, args, stream):
+    """The precision form: FP32 contraction on the matrix cores, FP32 element matrices and
+    values (sums of a row in FP64, rounded once); geometry and pointwise blocks in FP64."""
+    import torch
+
+    from hippymfem_amd import ADJOINT, STATE
+
+    blk = prob.block_assembly(ADJOINT, STATE, precision="fp32")
+    for _ in range(args.warmup):
+        blk.assemble(loc)
+    torch.cuda.synchronize()
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ev[k][0].record(stream)
+        blk.element_matrices(loc)
+        ev[k][1].record(stream)
+        blk.scatter()
+        ev[k][2].record(stream)
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / args.steps
+    tk = np.mean([ev[k][0].elapsed_time(ev[k][1]) for k in range(args.steps)])
+    ts = np.mean([ev[k][1].elapsed_time(ev[k][2]) for k in range(args.steps)])
+    d64 = blk64.data
+    d32 = blk.data.double()
+    err_max = float((d32 - d64).abs().max() / d64.abs().max())
+    err_fro = float(torch.linalg.vector_norm(d32 - d64) / torch.linalg.vector_norm(d64))
+    return {"ms_per_step": ms, "value": blk.pattern.nnz / (ms * 1e-3), "unit": "nnz/s",
+            "element_kernel_ms": tk, "scatter_ms": ts, "max_rel_err_vs_fp64": err_max,
+            "fro_rel_err_vs_fp64": err_fro}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -142,6 +174,7 @@ def main():
         "phases": {"element_kernel_ms": 1e3 * t_kernel, "scatter_ms": 1e3 * t_scatter,
                    "scatter_GBs": scatter_gbs, "element_kernel_TFLOPs": kernel_tf},
     }
+    out["fp32_form"] = fp32_form(prob, loc, blk, args, stream)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n)
     print(json.dumps(out), flush=True)

@@ -34,7 +34,8 @@ class HmProblem(ctypes.Structure):
     _fields_ = [("ne", I64), ("nq", I32), ("nd_u", I32), ("nd_m", I32), ("density", I32),
                 ("f", DBL), ("verts", P), ("elem_verts", P), ("edofs_u", P), ("edofs_m", P),
                 ("tab_u", P), ("tab_m", P), ("tab_x", P), ("wq", P), ("u", P), ("m", P),
-                ("p", P)]
+                ("p", P), ("nq1", I32), ("n1_u", I32), ("n1_m", I32), ("reserved", I32),
+                ("tab1_u", P), ("tab1_m", P), ("lex_u", P), ("lex_m", P)]
 
 
 class HmScatterPlan(ctypes.Structure):

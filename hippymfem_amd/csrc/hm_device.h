@@ -22,6 +22,10 @@ inline int hip_check(const char* what) {
 // The contraction on the matrix cores, FP64 or FP32 (hm_mfma.hip).
 int hm_element_matrices_mfma(const hm_problem* P, int i, int j, int64_t e0, int64_t e1,
                              int precision, void* out, void* stream);
+// The sum-factorized contraction for tensor-product hexahedra (hm_sumfact.hip); returns
+// HM_ERR_UNSUPPORTED when the problem has no tensor structure.
+int hm_element_matrices_sumfact(const hm_problem* P, int i, int j, int64_t e0, int64_t e1,
+                                int precision, void* out, void* stream);
 // BF16 / FP16 products with FP32 accumulation (hm_precision.hip).
 int hm_element_matrices_lowp(const hm_problem* P, int i, int j, int64_t e0, int64_t e1,
                              int precision, void* out, void* stream);

@@ -220,11 +220,17 @@ extern "C" int hm_element_matrices(const hm_problem* P, int i, int j, int64_t e0
   if (e1 == e0) return HM_OK;
   const int nd_ti = i == HM_PARAMETER ? P->nd_m : P->nd_u;
   const int nd_cj = j == HM_PARAMETER ? P->nd_m : P->nd_u;
+  // route: "sumfact" (default where the problem has tensor structure), "mfma", "valu"
   const char* route = getenv("HM_CONTRACTION");
   const bool valu = route && route[0] == 'v';
+  const bool mfma = route && route[0] == 'm';
   if (precision == HM_BF16 || precision == HM_FP16)
     return hm_element_matrices_lowp(P, i, j, e0, e1, precision, out, stream);
   HM_REQUIRE(precision == HM_FP64 || precision == HM_FP32, "unknown precision %d", precision);
+  if (!valu && !mfma) {
+    const int rc = hm_element_matrices_sumfact(P, i, j, e0, e1, precision, out, stream);
+    if (rc != HM_ERR_UNSUPPORTED) return rc;
+  }
   if (nd_ti <= 32 && nd_cj <= 32 && (!valu || precision == HM_FP32))
     return hm_element_matrices_mfma(P, i, j, e0, e1, precision, out, stream);
   HM_REQUIRE(precision == HM_FP64, "the FP32 contraction needs at most 32 dofs per element");

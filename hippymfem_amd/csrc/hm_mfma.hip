@@ -8,7 +8,8 @@
 // v_mfma_f64_16x16x4f64 (FP64) or v_mfma_f32_16x16x4f32 (FP32):
 //   A[i][k] (16 x 4): lane l holds A[l % 16][l / 16]   -- Psi_t, constant, in LDS
 //   B[k][j] (4 x 16): lane l holds B[l / 16][l % 16]   -- T, per element, in LDS
-//   D[i][j] (16x16): lane l, register v holds D[4 (l / 16) + v][l % 16].
+//   D[i][j] (16x16): lane l, register v holds D[row(l / 16, v)][l % 16], with
+//     row(g, v) = g + 4 v for the FP64 instruction and 4 g + v for the FP32 one.
 // The geometry of the element map (J, det J, J^-1), the fields and the pointwise blocks D^
 // are FP64 whatever the precision of the contraction; T is rounded to it.
 #include <hip/hip_runtime.h>
@@ -31,6 +32,7 @@ struct Mfma<double> {
   __device__ static inline acc op(double a, double b, acc c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  __device__ static inline int row(int g, int v) { return g + 4 * v; }
 };
 template <>
 struct Mfma<float> {
@@ -39,6 +41,7 @@ struct Mfma<float> {
   __device__ static inline acc op(float a, float b, acc c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  __device__ static inline int row(int g, int v) { return 4 * g + v; }
 };
 
 // Shape of one block's contraction.
@@ -153,7 +156,7 @@ __global__ void __launch_bounds__(256) k_element_mfma(hm_problem P, int s, int j
         for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
-            const int row = mt * 16 + 4 * g + v, col = nt * 16 + c16;
+            const int row = mt * 16 + Mfma<VT>::row(g, v), col = nt * 16 + c16;
             if (row < nd_t && col < nd_c) Eo[row * nd_c + col] = acc[mt][nt][v];
           }
     }

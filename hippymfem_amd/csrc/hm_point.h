@@ -21,7 +21,7 @@ struct PointState {
 
 __device__ inline void point_state(const hm_problem& P, int q, const double* Xe,
                                    const double* ue, const double* me, const double* pe,
-                                   PointState& S) {
+                                   PointState& S, bool fields = true) {
   const int nq = P.nq;
   // J[d][k] = sum_v X_v[d] dN_v/dxi_k (trilinear map)
   double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
@@ -51,7 +51,7 @@ __device__ inline void point_state(const hm_problem& P, int q, const double* Xe,
   const double* Gu = P.tab_u + (size_t)nq * nu + (size_t)q * nu * 3;
   const double* Nm = P.tab_m + (size_t)q * nm;
   double uq = 0, mq = 0, ru[3] = {0, 0, 0}, rp[3] = {0, 0, 0};
-  for (int a = 0; a < nu; ++a) {
+  for (int a = 0; a < (fields ? nu : 0); ++a) {
     uq = fma(Nu[a], ue[a], uq);
     for (int k = 0; k < 3; ++k) {
       ru[k] = fma(Gu[a * 3 + k], ue[a], ru[k]);

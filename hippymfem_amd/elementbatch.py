@@ -91,6 +91,8 @@ class ElementGroup:
         self.degree = int(quadrature_degree)
         self.points, self.w = integration_rule(self.degree)
         self.nq = self.points.shape[0]
+        self.nq1 = int(self.degree) // 2 + 1
+        self.points1, _ = gauss_legendre(self.nq1)
         self.ne = mesh.ne
         self.elems = np.arange(mesh.ne)
         _, self.Gx = vertex_basis(self.points)
@@ -121,6 +123,15 @@ class SpaceTables:
         self.vdim = 1
         self.N, self.G = basis_tables(space.order, group.points, space.local_nodes)
         self.edofs = space.edofs
+        # tensor structure: the 1-D basis at the 1-D points and the local dof of every
+        # tensor index a1 + n1 (a2 + n1 a3)
+        self.n1 = space.order + 1
+        V, D = _lagrange(space.order, group.points1)                 # (nq1, n1)
+        self.tab1 = np.ascontiguousarray(np.stack([V.T, D.T]))        # (2, n1, nq1)
+        idx = np.rint(np.asarray(space.local_nodes) * space.order).astype(np.int64)
+        tensor = idx[:, 0] + self.n1 * (idx[:, 1] + self.n1 * idx[:, 2])
+        self.lex = np.empty(self.nd, np.int32)
+        self.lex[tensor] = np.arange(self.nd, dtype=np.int32)
         self.signs = np.ones((1, 1))
         self._dev = None
 
@@ -128,7 +139,9 @@ class SpaceTables:
         if self._dev is None:
             tab = np.concatenate([self.N.reshape(-1), self.G.reshape(-1)])
             self._dev = dict(tab=_dev(tab, torch.float64),
-                             edofs=_dev(self.edofs, torch.int32))
+                             edofs=_dev(self.edofs, torch.int32),
+                             tab1=_dev(self.tab1, torch.float64),
+                             lex=_dev(self.lex, torch.int32))
         return self._dev
 
 

@@ -73,7 +73,11 @@ class QuadratureKernel:
             verts=_lib.ptr(g["verts"]), elem_verts=_lib.ptr(g["elem_verts"]),
             edofs_u=_lib.ptr(tu["edofs"]), edofs_m=_lib.ptr(tm["edofs"]),
             tab_u=_lib.ptr(tu["tab"]), tab_m=_lib.ptr(tm["tab"]), tab_x=_lib.ptr(g["tab_x"]),
-            wq=_lib.ptr(g["wq"]), u=_lib.ptr(u), m=_lib.ptr(m), p=_lib.ptr(p))
+            wq=_lib.ptr(g["wq"]), u=_lib.ptr(u), m=_lib.ptr(m), p=_lib.ptr(p),
+            nq1=self.group.nq1 if self.group.nq1 ** 3 == self.group.nq else 0,
+            n1_u=self.tables[STATE].n1, n1_m=self.tables[PARAMETER].n1,
+            tab1_u=_lib.ptr(tu["tab1"]), tab1_m=_lib.ptr(tm["tab1"]),
+            lex_u=_lib.ptr(tu["lex"]), lex_m=_lib.ptr(tm["lex"]))
         self._keep = (u, m, p, g, tu, tm)
         return P, (u, m, p)
 

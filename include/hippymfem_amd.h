@@ -73,6 +73,16 @@ typedef struct hm_problem {
   const double* u;
   const double* m;
   const double* p;
+  /* Tensor-product structure, for the sum-factorized contraction (nq1 = 0: none).
+   *   nq1          Gauss points per direction (nq = nq1^3, point (i, j, k) at i + nq1 (j + nq1 k))
+   *   n1_u, n1_m   1-D nodes per direction of the two spaces (order + 1)
+   *   tab1_u/_m    1-D basis at the 1-D points: [2][n1][nq1], values then derivatives
+   *   lex_u/_m     local dof of the tensor index a1 + n1 (a2 + n1 a3), (n1^3,) int32 */
+  int32_t nq1, n1_u, n1_m, reserved;
+  const double* tab1_u;
+  const double* tab1_m;
+  const int32_t* lex_u;
+  const int32_t* lex_m;
 } hm_problem;
 
 /* Element matrices of block (i, j) for elements [e0, e1): out (e1-e0, nd_i, nd_j), fp64

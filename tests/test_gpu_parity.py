@@ -219,23 +219,28 @@ def test_full_size_properties(gpu):
     assert rel(Eg, P.block(ADJOINT, STATE, elems)) <= TOL
 
 
+@pytest.mark.parametrize("numbering", ["lattice", "mfem"])
 @pytest.mark.parametrize("nonlinear", [False, True])
-def test_mfma_route_equals_valu_route(gpu, nonlinear, monkeypatch):
-    """The matrix-core contraction and the plain FMA one agree to round-off."""
-    W, _ = pair(4, perturb=0.2, nonlinear=nonlinear)
+def test_contraction_routes_agree(gpu, numbering, nonlinear, monkeypatch):
+    """The sum-factorized, matrix-core and plain FMA contractions agree to round-off, and
+    with the oracle."""
+    W, P = pair(4, perturb=0.2, nonlinear=nonlinear, numbering=numbering)
     blocks = BLOCKS + ([(STATE, STATE)] if nonlinear else [])
     for i, j in blocks:
-        Em = W.kernel.element_matrices(i, j, W.locals())[0].cpu().numpy()
-        monkeypatch.setenv("HM_CONTRACTION", "valu")
-        Ev = W.kernel.element_matrices(i, j, W.locals())[0].cpu().numpy()
+        Eo = P.block(i, j)
+        for route in ("sumfact", "mfma", "valu"):
+            monkeypatch.setenv("HM_CONTRACTION", route)
+            E = W.kernel.element_matrices(i, j, W.locals())[0].cpu().numpy()
+            assert rel(E, Eo) <= TOL, (route, i, j, rel(E, Eo))
         monkeypatch.delenv("HM_CONTRACTION")
-        assert rel(Em, Ev) <= TOL, (i, j)
 
 
 @pytest.mark.parametrize("nonlinear", [False, True])
-def test_fp32_contraction_error(gpu, nonlinear):
-    """The FP32 matrix-core form is within 1e-6 of the FP64 oracle, element matrices and
-    assembled blocks (values accumulated from FP32 entries)."""
+@pytest.mark.parametrize("route", ["sumfact", "mfma"])
+def test_fp32_contraction_error(gpu, nonlinear, route, monkeypatch):
+    """The FP32 form (sum-factorized or matrix-core) is within 1e-6 of the FP64 oracle,
+    element matrices and assembled blocks (values accumulated from FP32 entries)."""
+    monkeypatch.setenv("HM_CONTRACTION", route)
     W, P = pair(6, perturb=0.2, nonlinear=nonlinear)
     blocks = BLOCKS + ([(STATE, STATE)] if nonlinear else [])
     for i, j in blocks:
