@@ -19,16 +19,10 @@ struct PointState {
   double uq;            // value of u
 };
 
-__device__ inline void point_state(const hm_problem& P, int q, const double* Xe,
-                                   const double* ue, const double* me, const double* pe,
-                                   PointState& S, bool fields = true) {
-  const int nq = P.nq;
-  // J[d][k] = sum_v X_v[d] dN_v/dxi_k (trilinear map)
-  double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-  const double* gx = P.tab_x + (size_t)q * 24;
-  for (int v = 0; v < 8; ++v)
-    for (int d = 0; d < 3; ++d)
-      for (int k = 0; k < 3; ++k) J[d][k] = fma(Xe[v * 3 + d], gx[v * 3 + k], J[d][k]);
+// The point state from the Jacobian of the element map J[d][k] = dx_d / dxi_k and the
+// fields' values and reference gradients at the point.
+__device__ inline void point_finish(const hm_problem& P, int q, const double J[3][3], double uq,
+                                    const double ru[3], const double rp[3], double mq, PointState& S) {
   const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
   const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
   const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
@@ -45,20 +39,6 @@ __device__ inline void point_state(const hm_problem& P, int q, const double* Xe,
   S.Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
   S.Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
   S.w = P.wq[q] * fabs(det);
-  // fields: values and reference gradients, then physical gradients
-  const int nu = P.nd_u, nm = P.nd_m;
-  const double* Nu = P.tab_u + (size_t)q * nu;
-  const double* Gu = P.tab_u + (size_t)nq * nu + (size_t)q * nu * 3;
-  const double* Nm = P.tab_m + (size_t)q * nm;
-  double uq = 0, mq = 0, ru[3] = {0, 0, 0}, rp[3] = {0, 0, 0};
-  for (int a = 0; a < (fields ? nu : 0); ++a) {
-    uq = fma(Nu[a], ue[a], uq);
-    for (int k = 0; k < 3; ++k) {
-      ru[k] = fma(Gu[a * 3 + k], ue[a], ru[k]);
-      rp[k] = fma(Gu[a * 3 + k], pe[a], rp[k]);
-    }
-  }
-  for (int a = 0; a < nm; ++a) mq = fma(Nm[a], me[a], mq);
   for (int d = 0; d < 3; ++d) {
     S.gu[d] = S.Ji[0][d] * ru[0] + S.Ji[1][d] * ru[1] + S.Ji[2][d] * ru[2];
     S.gp[d] = S.Ji[0][d] * rp[0] + S.Ji[1][d] * rp[1] + S.Ji[2][d] * rp[2];
@@ -75,6 +55,71 @@ __device__ inline void point_state(const hm_problem& P, int q, const double* Xe,
     S.g1 = 0.0;
     S.g2 = 0.0;
   }
+}
+
+// tab_x / tab_m may point at LDS copies of the problem's tables (the default: the
+// problem's own, in global memory).
+__device__ inline void point_state(const hm_problem& P, int q, const double* Xe,
+                                   const double* ue, const double* me, const double* pe,
+                                   PointState& S, bool fields = true,
+                                   const double* tab_x = nullptr, const double* tab_m = nullptr) {
+  const int nq = P.nq;
+  // J[d][k] = sum_v X_v[d] dN_v/dxi_k (trilinear map)
+  double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  const double* gx = (tab_x ? tab_x : P.tab_x) + (size_t)q * 24;
+  for (int v = 0; v < 8; ++v)
+    for (int d = 0; d < 3; ++d)
+      for (int k = 0; k < 3; ++k) J[d][k] = fma(Xe[v * 3 + d], gx[v * 3 + k], J[d][k]);
+  const int nu = P.nd_u, nm = P.nd_m;
+  const double* Nu = P.tab_u + (size_t)q * nu;
+  const double* Gu = P.tab_u + (size_t)nq * nu + (size_t)q * nu * 3;
+  const double* Nm = (tab_m ? tab_m : P.tab_m) + (size_t)q * nm;
+  double uq = 0, mq = 0, ru[3] = {0, 0, 0}, rp[3] = {0, 0, 0};
+  for (int a = 0; a < (fields ? nu : 0); ++a) {
+    uq = fma(Nu[a], ue[a], uq);
+    for (int k = 0; k < 3; ++k) {
+      ru[k] = fma(Gu[a * 3 + k], ue[a], ru[k]);
+      rp[k] = fma(Gu[a * 3 + k], pe[a], rp[k]);
+    }
+  }
+  for (int a = 0; a < nm; ++a) mq = fma(Nm[a], me[a], mq);
+  point_finish(P, q, J, uq, ru, rp, mq, S);
+}
+
+// The point state of element e at point q with every operand read from global memory as it
+// is used (no per-thread staging arrays).
+__device__ inline void point_state_stream(const hm_problem& P, int64_t e, int q, bool fields, PointState& S) {
+  const int nq = P.nq;
+  double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  const double* gx = P.tab_x + (size_t)q * 24;
+  for (int v = 0; v < 8; ++v) {
+    const int32_t iv = P.elem_verts[e * 8 + v];
+    const double x0 = P.verts[(size_t)iv * 3], x1 = P.verts[(size_t)iv * 3 + 1], x2 = P.verts[(size_t)iv * 3 + 2];
+    for (int k = 0; k < 3; ++k) {
+      const double g = gx[v * 3 + k];
+      J[0][k] = fma(x0, g, J[0][k]);
+      J[1][k] = fma(x1, g, J[1][k]);
+      J[2][k] = fma(x2, g, J[2][k]);
+    }
+  }
+  const int nu = P.nd_u, nm = P.nd_m;
+  double uq = 0, mq = 0, ru[3] = {0, 0, 0}, rp[3] = {0, 0, 0};
+  if (fields) {
+    const double* Nu = P.tab_u + (size_t)q * nu;
+    const double* Gu = P.tab_u + (size_t)nq * nu + (size_t)q * nu * 3;
+    for (int a = 0; a < nu; ++a) {
+      const int32_t d = P.edofs_u[e * nu + a];
+      const double ua = P.u[d], pa = P.p[d];
+      uq = fma(Nu[a], ua, uq);
+      for (int k = 0; k < 3; ++k) {
+        ru[k] = fma(Gu[a * 3 + k], ua, ru[k]);
+        rp[k] = fma(Gu[a * 3 + k], pa, rp[k]);
+      }
+    }
+  }
+  const double* Nm = P.tab_m + (size_t)q * nm;
+  for (int a = 0; a < nm; ++a) mq = fma(Nm[a], P.m[P.edofs_m[e * nm + a]], mq);
+  point_finish(P, q, J, uq, ru, rp, mq, S);
 }
 
 // The pointwise block of test slot s and trial slot j, in the reference-gradient basis,
