@@ -57,24 +57,30 @@ __device__ inline void point_finish(const hm_problem& P, int q, const double J[3
   }
 }
 
-// tab_x / tab_m may point at LDS copies of the problem's tables (the default: the
+// tab_x / tab_m / tab_u may point at LDS copies of the problem's tables (the default: the
 // problem's own, in global memory).
 __device__ inline void point_state(const hm_problem& P, int q, const double* Xe,
                                    const double* ue, const double* me, const double* pe,
                                    PointState& S, bool fields = true,
-                                   const double* tab_x = nullptr, const double* tab_m = nullptr) {
+                                   const double* tab_x = nullptr, const double* tab_m = nullptr,
+                                   const double* tab_u = nullptr) {
   const int nq = P.nq;
   // J[d][k] = sum_v X_v[d] dN_v/dxi_k (trilinear map)
   double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
   const double* gx = (tab_x ? tab_x : P.tab_x) + (size_t)q * 24;
+  // loops unrolled by two: fully unrolled, every operand load is in flight at once and the
+  // registers they take limit the wavefronts a compute unit holds
+#pragma unroll 2
   for (int v = 0; v < 8; ++v)
     for (int d = 0; d < 3; ++d)
       for (int k = 0; k < 3; ++k) J[d][k] = fma(Xe[v * 3 + d], gx[v * 3 + k], J[d][k]);
   const int nu = P.nd_u, nm = P.nd_m;
-  const double* Nu = P.tab_u + (size_t)q * nu;
-  const double* Gu = P.tab_u + (size_t)nq * nu + (size_t)q * nu * 3;
+  const double* tu = tab_u ? tab_u : P.tab_u;
+  const double* Nu = tu + (size_t)q * nu;
+  const double* Gu = tu + (size_t)nq * nu + (size_t)q * nu * 3;
   const double* Nm = (tab_m ? tab_m : P.tab_m) + (size_t)q * nm;
   double uq = 0, mq = 0, ru[3] = {0, 0, 0}, rp[3] = {0, 0, 0};
+#pragma unroll 2
   for (int a = 0; a < (fields ? nu : 0); ++a) {
     uq = fma(Nu[a], ue[a], uq);
     for (int k = 0; k < 3; ++k) {

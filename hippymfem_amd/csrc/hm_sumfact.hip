@@ -1,4 +1,4 @@
-// Sum-factorized element contraction for tensor-product hexahedra, in two kernels.
+// Sum-factorized element matrices for tensor-product hexahedra, one kernel.
 //
 // The element matrix of a block is E[a][b] = sum_q sum_(r,c) D^[q][r][c] Psi_r(a, q) Psi_c(b, q),
 // r, c in {value, d/dxi_1, d/dxi_2, d/dxi_3}.  On a tensor element every Psi factorizes over
@@ -10,12 +10,12 @@
 // diffusion density it takes 17.5k multiply-adds an element, or 7k where the element is
 // axis-aligned, against 59k for the dense contraction.
 //
-// Kernel 1 (k_point_pairs): one thread per (element, point) computes, in FP64, the geometry of
-// the element map, the fields and the block's pointwise pairs W[e][p][q] (D^ of pair p at point
-// q) into a workspace.  Kernel 2 (k_sumfact_contract): one wavefront per element reads its
-// pairs, leaves out the pairs that are round-off on this element, runs stage 1 into LDS and
-// stages 2 and 3 in registers, and writes the element matrix.  Elements go in chunks whose workspace
-// (128 MiB) stays in the last-level cache between the two kernels.
+// A wavefront holds 64 / (NT NC NT) elements.  The lanes of an element first stage its vertex
+// coordinates and dof values in LDS, then act as its quadrature points and write the block's
+// pointwise pairs W[p][q] (D^ of pair p at point q, FP64) to LDS, then act as its (a1, b1, a2)
+// contraction tasks: each runs stages 1 and 2 for its (a1, b1, a2) in registers and stage 3
+// into its NC NT NC entries (b2, a3, b3) of the element matrix, which it writes to memory.
+// The pointwise pairs never leave the compute unit.
 #include <hip/hip_runtime.h>
 
 #include "hm_common.h"
@@ -28,55 +28,11 @@ constexpr int SF_MAXPAIR = 16;
 
 struct SumfactPlan {
   int npair;
+  int kmask;                           // bit 4 r + c set for every pair, pairs in the order of k
   int pr[SF_MAXPAIR], pc[SF_MAXPAIR];  // (test part r, trial part c) of each pair
   int tmask[4];                        // pairs of each third-direction type (r == 3, c == 3)
   int need_fields;                     // the pointwise blocks read u or p
 };
-
-// Kernel 1: thread (element e, point q) -> W[(e - e0) * npair + p][q].
-__global__ void __launch_bounds__(256) k_point_pairs(hm_problem P, int s, int j, int64_t e0, int64_t e1,
-                                                     SumfactPlan SP, double* __restrict__ W) {
-  // 27 points (the sum-factorized route's rule); a chunk holds fewer than 2^31 points
-  const int n = (int)((e1 - e0) * 27);
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
-    const int el = t / 27, q = t - 27 * (t / 27);
-    const int64_t e = e0 + el;
-    PointState st;
-    point_state_stream(P, e, q, SP.need_fields != 0, st);
-    double Dq[4][4];
-    point_block(s, j, st, Dq);
-    double* w = W + (size_t)el * SP.npair * 27 + q;
-    for (int p = 0; p < SP.npair; ++p) {
-      const int rr = SP.pr[p], cc = SP.pc[p];
-      double v = 0.0;
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) v = (r == rr && c == cc) ? Dq[r][c] : v;
-      w[p * 27] = v;
-    }
-  }
-}
-
-// The rows of a 1-D table for derivative flag d, from LDS.
-template <typename CT, int N>
-__device__ inline void tab_rows(const double* tab, int d, CT out[N][3]) {
-#pragma unroll
-  for (int n = 0; n < N; ++n)
-#pragma unroll
-    for (int q = 0; q < 3; ++q) out[n][q] = (CT)tab[(d * N + n) * 3 + q];
-}
-
-// acc[a][b] += sum_q X[a][q] Y[b][q] z[q]
-template <typename CT, int NT, int NC>
-__device__ inline void sf_accum(const CT X[NT][3], const CT Y[NC][3], const CT z[3], CT acc[NT][NC]) {
-#pragma unroll
-  for (int a = 0; a < NT; ++a) {
-    const CT x0 = X[a][0] * z[0], x1 = X[a][1] * z[1], x2 = X[a][2] * z[2];
-#pragma unroll
-    for (int b = 0; b < NC; ++b) acc[a][b] += x0 * Y[b][0] + x1 * Y[b][1] + x2 * Y[b][2];
-  }
-}
 
 // A pair whose pointwise weight stays below this fraction of the element's largest is left out
 // of that element's contraction: its share of every entry is below the rounding of the sum.
@@ -84,217 +40,150 @@ __device__ inline void sf_accum(const CT X[NT][3], const CT Y[NC][3], const CT z
 // which are round-off there.
 #define SF_DROP 1e-16
 
-constexpr int SF_WPB = 4;  // wavefronts (elements in flight) per block
+constexpr int SF_WPB = 4;  // wavefronts a block
 
-template <typename CT, int NT, int NC>
-__global__ void __launch_bounds__(256) k_sumfact_contract(const double* __restrict__ W, int64_t ne_chunk,
-                                                          int s, int j, const hm_problem P, SumfactPlan SP,
-                                                          CT* __restrict__ out) {
-  constexpr int NQ = 3;
-  constexpr int TC = NT * NC;  // 1-D (test node, trial node) pairs
-  constexpr int nd_t = NT * NT * NT, nd_c = NC * NC * NC;
-  extern __shared__ double lds[];
-  const int npair = SP.npair;
-  // LDS: local dof maps, the plan, per-wave pair lists, the 1-D tables, then per wave S1 and
-  // the staged element matrix.
-  int* lext = (int*)lds;                  // [nd_t]
-  int* lexc = lext + 32;                  // [nd_c]
-  int* spr = lexc + 32;                   // [SF_MAXPAIR]
-  int* spc = spr + SF_MAXPAIR;
-  int* wap_all = spc + SF_MAXPAIR;        // [SF_WPB][SF_MAXPAIR]
-  double* tabt = (double*)(wap_all + SF_WPB * SF_MAXPAIR);  // [2][NT][3]
-  double* tabc = tabt + 6 * NT;                             // [2][NC][3]
-  unsigned long long* mxp_all = (unsigned long long*)(tabc + 6 * NC);  // [SF_WPB][SF_MAXPAIR]
-  const int nw = npair * 27;
-  double* Wl_all = (double*)(mxp_all + SF_WPB * SF_MAXPAIR);             // [SF_WPB][nw]
-  CT* sbase = (CT*)(Wl_all + SF_WPB * nw);
-  const int s1_words = npair * TC * 9;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  CT* S1 = sbase + (size_t)wave * s1_words;
-  int* wap = wap_all + wave * SF_MAXPAIR;
-  unsigned long long* mxp = mxp_all + wave * SF_MAXPAIR;
-  double* Wl = Wl_all + wave * nw;
-  {
-    const double* t1t = s == HM_PARAMETER ? P.tab1_m : P.tab1_u;
-    const double* t1c = j == HM_PARAMETER ? P.tab1_m : P.tab1_u;
-    for (int i = threadIdx.x; i < 6 * NT; i += blockDim.x) tabt[i] = t1t[i];
-    for (int i = threadIdx.x; i < 6 * NC; i += blockDim.x) tabc[i] = t1c[i];
-    const int32_t* lt = s == HM_PARAMETER ? P.lex_m : P.lex_u;
-    const int32_t* lc = j == HM_PARAMETER ? P.lex_m : P.lex_u;
-    for (int i = threadIdx.x; i < nd_t; i += blockDim.x) lext[i] = lt[i];
-    for (int i = threadIdx.x; i < nd_c; i += blockDim.x) lexc[i] = lc[i];
-    if (threadIdx.x < SF_MAXPAIR) {
-      spr[threadIdx.x] = SP.pr[threadIdx.x];
-      spc[threadIdx.x] = SP.pc[threadIdx.x];
-    }
-  }
-  __syncthreads();
-  // Each wavefront walks its elements with the next element's pairs in flight: they are
-  // loaded into registers (at most 7 a lane) while the current element is contracted.
-  constexpr int PRE = (SF_MAXPAIR * 27 + 63) / 64;
-  const int64_t stride = (int64_t)gridDim.x * SF_WPB;
-  int64_t el = (int64_t)blockIdx.x * SF_WPB + wave;
-  double pre[PRE];
-#pragma unroll
-  for (int k = 0; k < PRE; ++k) {
-    const int idx = lane + 64 * k;
-    pre[k] = (el < ne_chunk && idx < nw) ? W[el * nw + idx] : 0.0;
-  }
-  for (; el < ne_chunk; el += stride) {
-    // the element's pairs into LDS, each pair's largest magnitude on the way (the bits of a
-    // non-negative double order as the double does)
-    if (lane < SF_MAXPAIR) mxp[lane] = 0ull;
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-#pragma unroll
-    for (int k = 0; k < PRE; ++k) {
-      const int idx = lane + 64 * k;
-      if (idx < nw) {
-        Wl[idx] = pre[k];
-        atomicMax(&mxp[idx / 27], (unsigned long long)__double_as_longlong(fabs(pre[k])));
-      }
-    }
-    // the next element's pairs, in flight from here on
-    {
-      const int64_t nx = el + stride;
-#pragma unroll
-      for (int k = 0; k < PRE; ++k) {
-        const int idx = lane + 64 * k;
-        pre[k] = (nx < ne_chunk && idx < nw) ? W[nx * nw + idx] : 0.0;
-      }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    const double* Wk = Wl;
-    // the pairs this element needs
-    const double mx = lane < npair ? __longlong_as_double((long long)mxp[lane]) : 0.0;
-    double top = mx;
-    for (int off = 8; off > 0; off >>= 1) top = fmax(top, __shfl_xor(top, off, 16));
-    top = __shfl(top, 0, 64);
-    const uint32_t live = (uint32_t)__ballot(lane < npair && mx > SF_DROP * top);
-    uint32_t mt[4];
-    int ct[4], ot[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      mt[t] = live & (uint32_t)SP.tmask[t];
-      ct[t] = __popc(mt[t]);
-    }
-    ot[0] = 0;
-    ot[1] = ct[0];
-    ot[2] = ct[0] + ct[1];
-    ot[3] = ct[0] + ct[1] + ct[2];
-    const int na = ot[3] + ct[3];
-    if (lane < npair && ((live >> lane) & 1)) {
-      const int t = ((spr[lane] == 3) << 1) | (spc[lane] == 3);
-      const uint32_t m = t == 0 ? mt[0] : t == 1 ? mt[1] : t == 2 ? mt[2] : mt[3];
-      const int o = t == 0 ? ot[0] : t == 1 ? ot[1] : t == 2 ? ot[2] : ot[3];
-      wap[o + __popc(m & ((1u << lane) - 1u))] = lane;
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    // stage 1: task (active pair, q2 + 3 q3): S1[ip][a1 b1][q23]
-    for (int task = lane; task < na * 9; task += 64) {
-      const int ip = task / 9, q23 = task % 9;
-      const int p = wap[ip];
-      const double* wp = Wk + p * 27 + 3 * q23;
-      const CT z[3] = {(CT)wp[0], (CT)wp[1], (CT)wp[2]};
-      CT Xs[NT][3], Ys[NC][3];
-      tab_rows<CT, NT>(tabt, spr[p] == 1, Xs);
-      tab_rows<CT, NC>(tabc, spc[p] == 1, Ys);
-      CT* o = S1 + ip * TC * 9 + q23;
-#pragma unroll
-      for (int a1 = 0; a1 < NT; ++a1) {
-        const CT x0 = Xs[a1][0] * z[0], x1 = Xs[a1][1] * z[1], x2 = Xs[a1][2] * z[2];
-#pragma unroll
-        for (int b1 = 0; b1 < NC; ++b1) o[(a1 * NC + b1) * 9] = x0 * Ys[b1][0] + x1 * Ys[b1][1] + x2 * Ys[b1][2];
-      }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    // stages 2 and 3: task (a1b1, a2b2); per type t of the third-direction factors,
-    //   S2_t[q3] = sum_{p of type t} sum_q2 X2_p(a2, q2) Y2_p(b2, q2) S1_p[a1b1][q2 + 3 q3]
-    // in registers, then E[a][b] += sum_q3 X3_t(a3, q3) Y3_t(b3, q3) S2_t[q3]; the pair and
-    // type loops are the same for every lane
-    CT* Eo = out + (size_t)el * nd_t * nd_c;
-    for (int task = lane; task < TC * TC; task += 64) {
-      const int ab1 = task / TC, ab2 = task % TC;
-      const int a2 = ab2 / NC, b2 = ab2 % NC;
-      CT acc[NT][NC];
-#pragma unroll
-      for (int a = 0; a < NT; ++a)
-#pragma unroll
-        for (int b = 0; b < NC; ++b) acc[a][b] = 0;
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        if (ct[t] == 0) continue;
-        CT s2[3] = {0, 0, 0};
-        for (int ip = ot[t]; ip < ot[t] + ct[t]; ++ip) {
-          const int p = wap[ip];
-          const double* xr = tabt + ((spr[p] == 2) * NT + a2) * 3;
-          const double* yr = tabc + ((spc[p] == 2) * NC + b2) * 3;
-          const CT f0 = (CT)xr[0] * (CT)yr[0], f1 = (CT)xr[1] * (CT)yr[1], f2 = (CT)xr[2] * (CT)yr[2];
-          const CT* s1 = S1 + (ip * TC + ab1) * 9;
-#pragma unroll
-          for (int q3 = 0; q3 < 3; ++q3) s2[q3] += f0 * s1[3 * q3] + f1 * s1[3 * q3 + 1] + f2 * s1[3 * q3 + 2];
-        }
-        CT Xs[NT][3], Ys[NC][3];
-        tab_rows<CT, NT>(tabt, t >> 1, Xs);
-        tab_rows<CT, NC>(tabc, t & 1, Ys);
-        sf_accum<CT, NT, NC>(Xs, Ys, s2, acc);
-      }
-      const int a1 = ab1 / NC, b1 = ab1 % NC;
-#pragma unroll
-      for (int a3 = 0; a3 < NT; ++a3) {
-        const int row = lext[a1 + NT * (a2 + NT * a3)];
-#pragma unroll
-        for (int b3 = 0; b3 < NC; ++b3) Eo[row * nd_c + lexc[b1 + NC * (b2 + NC * b3)]] = acc[a3][b3];
-      }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-  }
+// The lanes of a wavefront see each other's LDS writes: a wavefront's LDS operations complete
+// in order, so a fence against compiler reordering and a wavefront barrier suffice.
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
 }
 
-// Kernel 2, lane-mapped: lane (element, a1, b1, a2), NT*NC*NT lanes an element, no LDS.  The
-// lane runs stage 1 for its (a1, b1) and stage 2 for its a2 in registers, pair after pair,
-// and accumulates stage 3 into its NC*NT*NC entries (b2, a3, b3) of the element matrix.
+// LDS words of one element: its pairs W[npair][27], vertex coordinates [24], u and p dof
+// values [nd_u] each, m dof values [nd_m]; even, so every element starts 16-byte aligned.
+__host__ __device__ inline int sf_element_words(int npair, int nd_u, int nd_m) {
+  const int w = npair * 27 + 24 + 2 * nd_u + nd_m;
+  return (w + 1) & ~1;
+}
+
+// LDS words of the tables every element reads: the trilinear map's gradients [27][8][3], the
+// parameter basis [27][nd_m] and, when the fields are read, the state basis and its gradients
+// [27][nd_u] + [27][nd_u][3].
+__host__ __device__ inline int sf_table_words(int nd_u, int nd_m, bool fields) {
+  return 27 * 24 + 27 * nd_m + (fields ? 27 * nd_u * 4 : 0);
+}
+
 template <typename CT, int NT, int NC>
-__global__ void __launch_bounds__(256) k_sumfact_lanes(const double* __restrict__ W, int64_t ne_chunk,
-                                                       int s, int j, const hm_problem P, SumfactPlan SP,
-                                                       CT* __restrict__ out) {
-  constexpr int LPE = NT * NC * NT;      // lanes an element
-  constexpr int EPW = 64 / LPE;          // elements a wavefront
+__global__ void __launch_bounds__(256) k_sumfact(const hm_problem P, int s, int j, int64_t e0, int64_t e1,
+                                                 SumfactPlan SP, const double* __restrict__ t1t,
+                                                 const double* __restrict__ t1c, CT* __restrict__ out) {
+  constexpr int LPE = NT * NC * NT;          // lanes an element
+  constexpr int EPW = 64 / LPE;              // elements a wavefront
+  constexpr int NPF = (27 + LPE - 1) / LPE;  // dofs a lane fetches
   constexpr int nd_t = NT * NT * NT, nd_c = NC * NC * NC;
-  __shared__ double tabs[6 * (NT + NC)];
-  const double* t1t = s == HM_PARAMETER ? P.tab1_m : P.tab1_u;
-  const double* t1c = j == HM_PARAMETER ? P.tab1_m : P.tab1_u;
-  for (int i = threadIdx.x; i < 6 * NT; i += blockDim.x) tabs[i] = t1t[i];
-  for (int i = threadIdx.x; i < 6 * NC; i += blockDim.x) tabs[6 * NT + i] = t1c[i];
+  // the products of the 1-D tables, PT[dt][dc][a][b][q] = Xt_dt(a, q) Xc_dc(b, q), d the
+  // derivative flags: every factor of stages 1, 2 and 3
+  __shared__ double PT[4 * NT * NC * 3];
+  extern __shared__ double lds[];
+  const int npair = SP.npair, nu = P.nd_u, nm = P.nd_m;
+  const bool fields = SP.need_fields != 0;
+  double* TX = lds;
+  double* TM = TX + 27 * 24;
+  double* TU = TM + 27 * nm;
+  {
+    for (int i = threadIdx.x; i < 4 * NT * NC * 3; i += blockDim.x) {
+      const int q = i % 3, b = (i / 3) % NC, a = (i / (3 * NC)) % NT, dd = i / (3 * NC * NT);
+      PT[i] = t1t[((dd >> 1) * NT + a) * 3 + q] * t1c[((dd & 1) * NC + b) * 3 + q];
+    }
+    for (int i = threadIdx.x; i < 27 * 24; i += blockDim.x) TX[i] = P.tab_x[i];
+    for (int i = threadIdx.x; i < 27 * nm; i += blockDim.x) TM[i] = P.tab_m[i];
+    if (fields)
+      for (int i = threadIdx.x; i < 27 * nu * 4; i += blockDim.x) TU[i] = P.tab_u[i];
+  }
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int slot = lane / LPE, r = lane % LPE;
+  const bool in_use = slot < EPW;
   const int a1 = r % NT, b1 = (r / NT) % NC, a2 = r / (NT * NC);
-  // the 1-D tables B[d][n][q] at uniform LDS addresses (broadcast reads)
-  const double* Bt = tabs;
-  const double* Bc = tabs + 6 * NT;
   const int32_t* lt = s == HM_PARAMETER ? P.lex_m : P.lex_u;
   const int32_t* lc = j == HM_PARAMETER ? P.lex_m : P.lex_u;
-  const int npair = SP.npair;
-  const bool in_use = slot < EPW;
-  const int64_t stride = (int64_t)gridDim.x * (blockDim.x >> 6) * EPW;
-  for (int64_t base = ((int64_t)blockIdx.x * (blockDim.x >> 6) + wave) * EPW; base < ne_chunk; base += stride) {
+  double* Wl = lds + sf_table_words(nu, nm, fields) +
+               (size_t)(wave * EPW + (in_use ? slot : 0)) * sf_element_words(npair, nu, nm);
+  double* Xl = Wl + npair * 27;
+  double* Ul = Xl + 24;
+  double* Pl = Ul + nu;
+  double* Ml = Pl + nu;
+  const int64_t ne = e1 - e0;
+  const int64_t stride = (int64_t)gridDim.x * SF_WPB * EPW;
+  // the next element's vertex coordinates and dof values, in flight in registers while the
+  // current element is computed: lane r holds vertex r and dofs r, r + LPE, ...
+  double fx[3], fu[NPF], fp[NPF], fm[NPF];
+  auto fetch = [&](int64_t el) {
+    const bool ok = in_use && el < ne;
+    const int64_t e = e0 + (ok ? el : 0);
+    if (ok && r < 8) {
+      const int64_t iv = P.elem_verts[e * 8 + r];
+      fx[0] = P.verts[iv * 3];
+      fx[1] = P.verts[iv * 3 + 1];
+      fx[2] = P.verts[iv * 3 + 2];
+    }
+#pragma unroll
+    for (int k = 0; k < NPF; ++k) {
+      const int a = r + k * LPE;
+      if (ok && fields && a < nu) {
+        const int64_t d = P.edofs_u[e * nu + a];
+        fu[k] = P.u[d];
+        fp[k] = P.p[d];
+      }
+      if (ok && a < nm) fm[k] = P.m[P.edofs_m[e * nm + a]];
+    }
+  };
+  int64_t base = ((int64_t)blockIdx.x * SF_WPB + wave) * EPW;
+  fetch(base + slot);
+  for (; base < ne; base += stride) {
     // the table reads stay inside the loop (hoisted, the products would take the registers)
     asm volatile("" ::: "memory");
-    const int64_t e = base + slot;
-    const bool act = in_use && e < ne_chunk;
-    const double* We = W + (act ? e : 0) * npair * 27;
-    // the pairs this element needs: lane r measures pairs r and r + LPE against the element's
-    // largest pointwise weight
+    const int64_t el = base + slot;
+    const bool act = in_use && el < ne;
+    // 1. the element's vertex coordinates and dof values into LDS; the next element's fetched
+    if (act) {
+      if (r < 8) {
+        Xl[r * 3] = fx[0];
+        Xl[r * 3 + 1] = fx[1];
+        Xl[r * 3 + 2] = fx[2];
+      }
+#pragma unroll
+      for (int k = 0; k < NPF; ++k) {
+        const int a = r + k * LPE;
+        if (fields && a < nu) {
+          Ul[a] = fu[k];
+          Pl[a] = fp[k];
+        }
+        if (a < nm) Ml[a] = fm[k];
+      }
+    }
+    fetch(el + stride);
+    wave_sync();
+    // 2. lane = quadrature point: the pointwise pairs into LDS, in the order of k = 4 r + c
+    if (act)
+      for (int q = r; q < 27; q += LPE) {
+        PointState S;
+        point_state(P, q, Xl, Ul, Ml, Pl, S, fields, TX, TM, TU);
+        double D[4][4];
+        point_block(s, j, S, D);
+        int p = 0;
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+          if ((SP.kmask >> k) & 1) {
+            const double v = D[k >> 2][k & 3];
+            Wl[p * 27 + q] = v;
+            ++p;
+          }
+      }
+    wave_sync();
+    // 3. the pairs this element needs: lane r measures pairs r and r + LPE against the
+    // element's largest pointwise weight
     double m0 = 0.0, m1 = 0.0;
-    if (act && r < npair)
-      for (int q = 0; q < 27; ++q) m0 = fmax(m0, fabs(We[r * 27 + q]));
+    if (act && r < npair) {
+      double v[27];
+#pragma unroll
+      for (int q = 0; q < 27; ++q) v[q] = Wl[r * 27 + q];
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int q = 0; q < 27; ++q) m0 = fmax(m0, fabs(v[q]));
+    }
     if (act && LPE + r < npair)
-      for (int q = 0; q < 27; ++q) m1 = fmax(m1, fabs(We[(LPE + r) * 27 + q]));
+      for (int q = 0; q < 27; ++q) m1 = fmax(m1, fabs(Wl[(LPE + r) * 27 + q]));
     double top = 0.0;
     for (int p = 0; p < npair; ++p) {
       const double v0 = __shfl(m0, slot * LPE + p % LPE, 64), v1 = __shfl(m1, slot * LPE + p % LPE, 64);
@@ -306,6 +195,7 @@ __global__ void __launch_bounds__(256) k_sumfact_lanes(const double* __restrict_
     const int sh = in_use ? slot * LPE : 0;
     const uint64_t both = ((bal0 >> sh) & lpe_mask) | (((bal1 >> sh) & lpe_mask) << LPE);
     const uint32_t live = (uint32_t)both & ((1u << npair) - 1u);
+    // 4. lane = contraction task (a1, b1, a2)
     CT acc[NC][NT][NC];
 #pragma unroll
     for (int b2 = 0; b2 < NC; ++b2)
@@ -313,7 +203,7 @@ __global__ void __launch_bounds__(256) k_sumfact_lanes(const double* __restrict_
       for (int a3 = 0; a3 < NT; ++a3)
 #pragma unroll
         for (int b3 = 0; b3 < NC; ++b3) acc[b2][a3][b3] = 0;
-#pragma unroll
+#pragma unroll 1
     for (int t = 0; t < 4; ++t) {
       const uint32_t tm = live & (uint32_t)SP.tmask[t];
       if (!act || tm == 0) continue;
@@ -322,45 +212,64 @@ __global__ void __launch_bounds__(256) k_sumfact_lanes(const double* __restrict_
       for (int b2 = 0; b2 < NC; ++b2) s2[b2][0] = s2[b2][1] = s2[b2][2] = 0;
       for (int p = 0; p < npair; ++p) {
         if (!((tm >> p) & 1)) continue;
-        const int d1x = SP.pr[p] == 1, d1y = SP.pc[p] == 1, d2x = SP.pr[p] == 2, d2y = SP.pc[p] == 2;
-        const double* w = We + p * 27;
-        // stage 1: s1[q2 + 3 q3] = sum_q1 X(a1,q1) Y(b1,q1) W(q1, q2, q3)
-        CT f1[3];
+        const int d1 = ((SP.pr[p] == 1) << 1) | (SP.pc[p] == 1);
+        const int d2 = ((SP.pr[p] == 2) << 1) | (SP.pc[p] == 2);
+        // every LDS operand of the pair in flight at once (one wait, not one a use)
+        const double* w = Wl + p * 27;
+        double wv[27];
 #pragma unroll
-        for (int q = 0; q < 3; ++q) f1[q] = (CT)(Bt[((d1x ? NT : 0) + a1) * 3 + q] * Bc[((d1y ? NC : 0) + b1) * 3 + q]);
-        CT s1[9];
-#pragma unroll
-        for (int q23 = 0; q23 < 9; ++q23)
-          s1[q23] = f1[0] * (CT)w[3 * q23] + f1[1] * (CT)w[3 * q23 + 1] + f1[2] * (CT)w[3 * q23 + 2];
-        // stage 2: s2[b2][q3] += sum_q2 X(a2,q2) Y(b2,q2) s1[q2 + 3 q3]
-        CT x2[3];
-#pragma unroll
-        for (int q = 0; q < 3; ++q) x2[q] = (CT)Bt[((d2x ? NT : 0) + a2) * 3 + q];
+        for (int i = 0; i < 27; ++i) wv[i] = w[i];
+        const double* f1 = PT + ((d1 * NT + a1) * NC + b1) * 3;
+        const double g0d = f1[0], g1d = f1[1], g2d = f1[2];
+        double hd[NC][3];
 #pragma unroll
         for (int b2 = 0; b2 < NC; ++b2) {
-          CT f2[3];
-#pragma unroll
-          for (int q = 0; q < 3; ++q) f2[q] = x2[q] * (CT)Bc[((d2y ? NC : 0) + b2) * 3 + q];
-#pragma unroll
-          for (int q3 = 0; q3 < 3; ++q3)
-            s2[b2][q3] += f2[0] * s1[3 * q3] + f2[1] * s1[3 * q3 + 1] + f2[2] * s1[3 * q3 + 2];
+          const double* f2 = PT + ((d2 * NT + a2) * NC + b2) * 3;
+          hd[b2][0] = f2[0];
+          hd[b2][1] = f2[1];
+          hd[b2][2] = f2[2];
         }
-      }
-      // stage 3 for this type of third-direction factors
+        __builtin_amdgcn_sched_barrier(0);
+        const CT g0 = (CT)g0d, g1 = (CT)g1d, g2 = (CT)g2d;
+        // one q3 plane at a time:
+        //   stage 1: s1[q2] = sum_q1 X(a1, q1) Y(b1, q1) W(q1, q2, q3)
+        //   stage 2: s2[b2][q3] += sum_q2 X(a2, q2) Y(b2, q2) s1[q2]
 #pragma unroll
-      for (int a3 = 0; a3 < NT; ++a3)
+        for (int q3 = 0; q3 < 3; ++q3) {
+          CT s1[3];
 #pragma unroll
-        for (int b3 = 0; b3 < NC; ++b3) {
-          CT f3[3];
-#pragma unroll
-          for (int q = 0; q < 3; ++q) f3[q] = (CT)(Bt[((t >> 1) * NT + a3) * 3 + q] * Bc[((t & 1) * NC + b3) * 3 + q]);
+          for (int q2 = 0; q2 < 3; ++q2) {
+            const double* wr = wv + 3 * q2 + 9 * q3;
+            s1[q2] = fma(g2, (CT)wr[2], fma(g1, (CT)wr[1], g0 * (CT)wr[0]));
+          }
 #pragma unroll
           for (int b2 = 0; b2 < NC; ++b2)
-            acc[b2][a3][b3] += f3[0] * s2[b2][0] + f3[1] * s2[b2][1] + f3[2] * s2[b2][2];
+            s2[b2][q3] = fma((CT)hd[b2][2], s1[2], fma((CT)hd[b2][1], s1[1], fma((CT)hd[b2][0], s1[0], s2[b2][q3])));
         }
+      }
+      // stage 3 for this type of third-direction factors, the 1-D rows read as wavefront-uniform
+      // values from memory (scalar loads):
+      //   E[a3][b3] += sum_q3 Xt(a3, q3) (Yc(b3, q3) s2[b2][q3])
+      const double* bt = t1t + (t >> 1) * NT * 3;
+      const double* bc = t1c + (t & 1) * NC * 3;
+#pragma unroll
+      for (int b3 = 0; b3 < NC; ++b3) {
+        CT u[NC][3];
+#pragma unroll
+        for (int b2 = 0; b2 < NC; ++b2)
+#pragma unroll
+          for (int q = 0; q < 3; ++q) u[b2][q] = (CT)bc[b3 * 3 + q] * s2[b2][q];
+#pragma unroll
+        for (int a3 = 0; a3 < NT; ++a3) {
+          const CT k0 = (CT)bt[a3 * 3], k1 = (CT)bt[a3 * 3 + 1], k2 = (CT)bt[a3 * 3 + 2];
+#pragma unroll
+          for (int b2 = 0; b2 < NC; ++b2)
+            acc[b2][a3][b3] = fma(k2, u[b2][2], fma(k1, u[b2][1], fma(k0, u[b2][0], acc[b2][a3][b3])));
+        }
+      }
     }
     if (act) {
-      CT* Eo = out + (size_t)e * nd_t * nd_c;
+      CT* Eo = out + (size_t)el * nd_t * nd_c;
 #pragma unroll
       for (int a3 = 0; a3 < NT; ++a3) {
         CT* Er = Eo + lt[a1 + NT * (a2 + NT * a3)] * nd_c;
@@ -370,71 +279,44 @@ __global__ void __launch_bounds__(256) k_sumfact_lanes(const double* __restrict_
           for (int b3 = 0; b3 < NC; ++b3) Er[lc[b1 + NC * (b2 + NC * b3)]] = acc[b2][a3][b3];
       }
     }
+    // the LDS of this element is rewritten by the next
+    wave_sync();
   }
-}
-
-// Per-device workspace of the pointwise pairs, allocated on first use and kept for the life of
-// the process (128 MiB: a chunk of elements whose pairs stay in the last-level cache).
-constexpr size_t SF_WORKSPACE = size_t(128) << 20;
-
-static double* workspace() {
-  static double* ws[64] = {nullptr};
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
-  if (!ws[dev]) {
-    void* p = nullptr;
-    if (hipMalloc(&p, SF_WORKSPACE) != hipSuccess) return nullptr;
-    ws[dev] = (double*)p;
-  }
-  return ws[dev];
 }
 
 template <typename CT, int NT, int NC>
 static int launch_sumfact(const hm_problem& P, int i, int j, int64_t e0, int64_t e1, const SumfactPlan& SP,
                           void* out, hipStream_t st) {
-  constexpr int TC = NT * NC;
-  const size_t head = (64 + 2 * SF_MAXPAIR + SF_WPB * SF_MAXPAIR) * 4 + 6 * (NT + NC) * 8;
-  const size_t per_wave = (size_t)SP.npair * TC * 9 * sizeof(CT) + (size_t)SP.npair * 27 * 8 + SF_MAXPAIR * 8;
-  const size_t bytes = head + SF_WPB * per_wave;
+  constexpr int EPW = 64 / (NT * NC * NT);
+  const size_t bytes = ((size_t)sf_table_words(P.nd_u, P.nd_m, SP.need_fields != 0) +
+                        (size_t)SF_WPB * EPW * sf_element_words(SP.npair, P.nd_u, P.nd_m)) * sizeof(double);
   int limit = 64 * 1024, dev = 0;
   if (hipGetDevice(&dev) == hipSuccess) {
     int v = 0;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0) limit = v;
   }
-  if (bytes > (size_t)limit)
+  if (bytes + 4 * NT * NC * 3 * sizeof(double) > (size_t)limit)
     HM_FAIL(HM_ERR_UNSUPPORTED, "the sum-factorized contraction needs %zu bytes of LDS", bytes);
-  if (bytes > 64 * 1024) {
-    hipError_t err = hipFuncSetAttribute((const void*)k_sumfact_contract<CT, NT, NC>,
+  if (bytes > 48 * 1024) {
+    hipError_t err = hipFuncSetAttribute((const void*)k_sumfact<CT, NT, NC>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (err != hipSuccess) HM_FAIL(HM_ERR_HIP, "hipFuncSetAttribute: %s", hipGetErrorString(err));
   }
-  const char* lv = getenv("HM_SF_LANES");
-  const bool lanes = lv && lv[0] == '1';
-  double* W = workspace();
-  if (!W) HM_FAIL(HM_ERR_HIP, "the %zu-byte workspace of the contraction could not be allocated", SF_WORKSPACE);
-  const int64_t chunk = (int64_t)(SF_WORKSPACE / ((size_t)SP.npair * P.nq * sizeof(double)));
-  const size_t nd = (size_t)(NT * NT * NT) * (NC * NC * NC);
-  for (int64_t a = e0; a < e1; a += chunk) {
-    const int64_t b = a + chunk < e1 ? a + chunk : e1;
-    const int64_t nthreads = (b - a) * P.nq;
-    const int64_t nb = (nthreads + 255) / 256;
-    hipLaunchKernelGGL(k_point_pairs, dim3((unsigned)(nb < 1048576 ? nb : 1048576)), dim3(256), 0, st, P, i, j,
-                       a, b, SP, W);
-    if (lanes) {
-      constexpr int EPB = (64 / (NT * NC * NT)) * 4;  // elements a block of 4 wavefronts
-      const int64_t groups = (b - a + EPB - 1) / EPB;
-      const int grid = (int)(groups < 8192 ? groups : 8192);
-      hipLaunchKernelGGL((k_sumfact_lanes<CT, NT, NC>), dim3(grid), dim3(256), 0, st, W, b - a, i, j, P, SP,
-                         (CT*)out + (size_t)(a - e0) * nd);
-    } else {
-      // persistent: a few blocks per compute unit, each wavefront walking many elements
-      const int64_t groups = (b - a + SF_WPB - 1) / SF_WPB;
-      const int grid = (int)(groups < 1024 ? groups : 1024);
-      hipLaunchKernelGGL((k_sumfact_contract<CT, NT, NC>), dim3(grid), dim3(256), bytes, st, W, b - a, i, j, P,
-                         SP, (CT*)out + (size_t)(a - e0) * nd);
-    }
-  }
-  return hip_check("k_sumfact_contract");
+  // persistent: as many blocks as the device holds at once, each walking many elements (the
+  // tables are loaded once a block and the next element's data is in flight meanwhile)
+  int per_cu = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_sumfact<CT, NT, NC>, 256, bytes) !=
+          hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
+  const int64_t groups = (e1 - e0 + SF_WPB * EPW - 1) / (SF_WPB * EPW);
+  const int64_t full = (int64_t)per_cu * cus;
+  const int grid = (int)(groups < full ? groups : full);
+  hipLaunchKernelGGL((k_sumfact<CT, NT, NC>), dim3(grid), dim3(256), bytes, st, P, i, j, e0, e1, SP,
+                     (const double*)(i == HM_PARAMETER ? P.tab1_m : P.tab1_u),
+                     (const double*)(j == HM_PARAMETER ? P.tab1_m : P.tab1_u), (CT*)out);
+  return hip_check("k_sumfact");
 }
 
 }  // namespace hm
@@ -452,6 +334,8 @@ int hm_element_matrices_sumfact(const hm_problem* P, int i, int j, int64_t e0, i
   const int nt = i == HM_PARAMETER ? P->n1_m : P->n1_u;
   const int nc = j == HM_PARAMETER ? P->n1_m : P->n1_u;
   if (nt < 2 || nt > 3 || nc < 2 || nc > 3) HM_FAIL(HM_ERR_UNSUPPORTED, "1-D orders outside 1..2");
+  if (P->nd_u != P->n1_u * P->n1_u * P->n1_u || P->nd_m != P->n1_m * P->n1_m * P->n1_m)
+    HM_FAIL(HM_ERR_UNSUPPORTED, "the element dofs are not the tensor products of the 1-D nodes");
   SumfactPlan SP = {};
   // the nonzero (r, c) pairs of the block: the vv, vg, gv, gg parts of the density
   const bool nl = P->density == HM_DENSITY_NONLINEAR;
@@ -474,6 +358,7 @@ int hm_element_matrices_sumfact(const hm_problem* P, int i, int j, int64_t e0, i
       if (on) {
         SP.pr[np] = r;
         SP.pc[np] = c;
+        SP.kmask |= 1 << (4 * r + c);
         ++np;
       }
     }
