@@ -9,11 +9,14 @@ the timed region; the pattern build is one-time set-up, as in the reference's wa
 (``benchmarks/bench_assembly.py:10-17``).
 
 Prints one JSON line.  ``--gpus`` above 1 is refused: this path is measured on one GPU.
+``--copy-calibration`` runs a plain device copy of known size (the reference point for the
+HBM byte counters) and prints its size.
 """
 
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -22,7 +25,8 @@ import numpy as np
 METRIC = ("assembled CSR nnz/s + % HBM roofline at 64³ P2 (FP64 vs FP32-MFMA); "
           "p50/p99 latency")
 HBM_PEAK_GBS = 8000.0
-FP64_PEAK_TFLOPS = 78.6
+HERE = os.path.dirname(os.path.abspath(__file__))
+TRAFFIC_FILE = os.path.join(HERE, "profiles", "hbm_traffic_A_64cube.md")
 
 
 def parse():
@@ -32,7 +36,10 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=64, help="hexahedra per direction")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving sample")
     ap.add_argument("--cpu-n", type=int, default=12, help="mesh of the CPU baseline sample")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run a plain 1 GiB device copy five times and exit")
     return ap.parse_args()
 
 
@@ -78,36 +85,96 @@ def cpu_baseline(n):
                       "%.1f us/elem" % (n, P.mesh.ne, nnz, reps, 1e6 * dt / P.mesh.ne)}
 
 
+def timed_steps(blk, loc, steps, warmup, stream):
+    """Warm-up, then ``steps`` assemblies bracketed by synchronizations; HIP events on the
+    stream the kernels run on split every step into its element-kernel and scatter launches."""
+    import torch
+
+    for _ in range(warmup):
+        blk.assemble(loc, stream=stream)
+    torch.cuda.synchronize()
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+    t0 = time.perf_counter()
+    for k in range(steps):
+        ev[k][0].record(stream)
+        blk.element_matrices(loc, stream=stream)
+        ev[k][1].record(stream)
+        blk.scatter(stream=stream)
+        ev[k][2].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    tk = float(np.mean([ev[k][0].elapsed_time(ev[k][1]) for k in range(steps)])) * 1e-3
+    ts = float(np.mean([ev[k][1].elapsed_time(ev[k][2]) for k in range(steps)])) * 1e-3
+    return wall / steps, tk, ts
+
+
+def algorithmic_bytes(blk, W):
+    """HBM bytes a launch must move at the least (every operand read once, every result
+    written once).
+
+    element kernel: the element matrices written (8 or 4 bytes an entry), the element vertex
+    and dof tables and the vertex coordinates and dof values read (the fields only where the
+    block reads them); scatter: the element matrices read, one code byte an entry, the runs
+    (4 bytes an element row), the row tables (indptr, rptr 4 bytes, rowmode 1 byte a row) and
+    the values written."""
+    pat = blk.pattern
+    k = blk.kernel
+    esize = 8 if blk.precision == "fp64" else 4
+    vsize = blk.data.element_size()
+    ne, nd_t, nd_c = pat.ne, pat.nd_t, pat.nd_c
+    nent = ne * nd_t * nd_c
+    Vu, Vm = k.spaces[0], k.spaces[1]
+    fields = not (blk.i == 2 and blk.j == 0 and not k.density.nonlinear)
+    b_elem = (esize * nent + 4 * 8 * ne + 24 * W.mesh.nvert + 4 * ne * Vm.nd + 8 * Vm.ndof +
+              ((4 * ne * Vu.nd + 16 * Vu.ndof) if fields else 0))
+    b_scatter = esize * nent + nent + 4 * ne * nd_t + 9 * pat.nrow + vsize * pat.nnz
+    return b_elem, b_scatter
+
+
+def measured_traffic():
+    """HBM bytes a launch moved, read by the byte counters (``FETCH_SIZE`` + ``WRITE_SIZE``,
+    scaled by a plain copy's reading) from the committed profile summary; None without one."""
+    try:
+        text = open(TRAFFIC_FILE).read()
+    except OSError:
+        return {}
+    out = {}
+    for key in ("element_kernel", "scatter"):
+        m = re.search(r"^%s_traffic_bytes:\s*([0-9.eE+]+)" % key, text, re.M)
+        if m:
+            out[key] = float(m.group(1))
+    return out
+
+
 def fp32_form(prob, loc, blk64, args, stream):
-    """The precision form: FP32 contraction on the matrix cores, FP32 element matrices and
-    values (sums of a row in FP64, rounded once); geometry and pointwise blocks in FP64."""
+    """The precision form: FP32 contraction and element matrices, FP32 values (each sum in
+    FP64, rounded once); geometry and pointwise blocks in FP64.  Error against the FP64
+    values of the same step."""
     import torch
 
     from hippymfem_amd import ADJOINT, STATE
 
     blk = prob.block_assembly(ADJOINT, STATE, precision="fp32")
-    for _ in range(args.warmup):
-        blk.assemble(loc)
-    torch.cuda.synchronize()
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        ev[k][0].record(stream)
-        blk.element_matrices(loc)
-        ev[k][1].record(stream)
-        blk.scatter()
-        ev[k][2].record(stream)
-    torch.cuda.synchronize()
-    ms = 1e3 * (time.perf_counter() - t0) / args.steps
-    tk = np.mean([ev[k][0].elapsed_time(ev[k][1]) for k in range(args.steps)])
-    ts = np.mean([ev[k][1].elapsed_time(ev[k][2]) for k in range(args.steps)])
+    step, tk, ts = timed_steps(blk, loc, args.steps, args.warmup, stream)
     d64 = blk64.data
     d32 = blk.data.double()
     err_max = float((d32 - d64).abs().max() / d64.abs().max())
     err_fro = float(torch.linalg.vector_norm(d32 - d64) / torch.linalg.vector_norm(d64))
-    return {"ms_per_step": ms, "value": blk.pattern.nnz / (ms * 1e-3), "unit": "nnz/s",
-            "element_kernel_ms": tk, "scatter_ms": ts, "max_rel_err_vs_fp64": err_max,
-            "fro_rel_err_vs_fp64": err_fro}
+    return {"ms_per_step": 1e3 * step, "value": blk.pattern.nnz / step, "unit": "nnz/s",
+            "element_kernel_ms": 1e3 * tk, "scatter_ms": 1e3 * ts,
+            "max_rel_err_vs_fp64": err_max, "fro_rel_err_vs_fp64": err_fro}
+
+
+def copy_calibration():
+    import torch
+
+    n = 1 << 27                                                     # 2^27 doubles = 1 GiB
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "launches": 5}))
 
 
 def main():
@@ -116,6 +183,9 @@ def main():
         print("bench.py measures this path on one GPU; --gpus %d is not supported" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
     import torch
 
     from hippymfem_amd import ADJOINT, STATE
@@ -125,61 +195,50 @@ def main():
     dev = torch.device("cuda", 0)
     W = Workload(args.n)
     prob = PDEVariationalProblem(W.spaces, W.density, W.batches, W.bc)
-    x = [W.u, W.m, W.p]
     blk = prob.block_assembly(ADJOINT, STATE)
-    loc = [torch.as_tensor(v).to(dev) for v in x]
+    loc = [torch.as_tensor(v).to(dev) for v in (W.u, W.m, W.p)]
     stream = torch.cuda.current_stream()
-    for _ in range(args.warmup):
-        blk.assemble(loc)
-    torch.cuda.synchronize()
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        ev[k][0].record(stream)
-        blk.element_matrices(loc)
-        ev[k][1].record(stream)
-        blk.scatter()
-        ev[k][2].record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    t_kernel = np.mean([ev[k][0].elapsed_time(ev[k][1]) for k in range(args.steps)]) * 1e-3
-    t_scatter = np.mean([ev[k][1].elapsed_time(ev[k][2]) for k in range(args.steps)]) * 1e-3
-    ms = 1e3 * wall / args.steps
+    step, t_kernel, t_scatter = timed_steps(blk, loc, args.steps, args.warmup, stream)
     pat = blk.pattern
-    nnz, nent, ne = pat.nnz, pat.nent, pat.ne
-    nrow = pat.nrow
-    # algorithmic bytes of the scatter: E read, one code byte per entry, the runs, the
-    # row pointers (indptr, rptr, rowmode) and the values written
-    b_scatter = 8 * nent + 1 * nent + 4 * ne * pat.nd_t + (4 + 4 + 1) * nrow + 8 * nnz
-    flops_kernel = 2.0 * 27 * 27 * 81 * ne
-    scatter_gbs = b_scatter / t_scatter / 1e9
-    kernel_tf = flops_kernel / t_kernel / 1e12
-    if t_scatter >= t_kernel:
-        roof = {"kernel": "k_scatter_rows", "bound": "hbm", "achieved": scatter_gbs,
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": scatter_gbs / HBM_PEAK_GBS,
-                "traffic": None}
-    else:
-        roof = {"kernel": "k_element_matrices", "bound": "mfma", "achieved": kernel_tf,
-                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": kernel_tf / FP64_PEAK_TFLOPS, "traffic": None}
+    b_elem, b_scatter = algorithmic_bytes(blk, W)
+    traffic = measured_traffic()
+    kernels = {
+        "k_sumfact": {"ms": 1e3 * t_kernel, "bytes": b_elem,
+                      "GBs": b_elem / t_kernel / 1e9,
+                      "traffic": traffic.get("element_kernel")},
+        "k_scatter_rows": {"ms": 1e3 * t_scatter, "bytes": b_scatter,
+                           "GBs": b_scatter / t_scatter / 1e9,
+                           "traffic": traffic.get("scatter")},
+    }
+    dom = max(kernels, key=lambda k: kernels[k]["ms"])
+    d = kernels[dom]
+    roof = {"kernel": dom, "bound": "hbm", "achieved": d["GBs"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": d["GBs"] / HBM_PEAK_GBS, "traffic": d["traffic"]}
+    b_e2e = b_elem + b_scatter
     out = {
-        "metric": METRIC, "value": nnz / (ms * 1e-3), "unit": "nnz/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "metric": METRIC, "value": pat.nnz / step, "unit": "nnz/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic",
+        "data": "synthetic (seeded fields, unit-cube mesh)",
         "config": {"workload": "%d^3 hex P2 Jacobian A, exp(m) diffusion, FP64, 3x3x3 Gauss, "
                                "Dirichlet x=0,1 folded" % args.n,
-                   "elements": ne, "nnz": nnz, "entries": nent, "parallelism": "1 GPU"},
+                   "elements": pat.ne, "nnz": pat.nnz, "entries": pat.ne * pat.nd_t * pat.nd_c,
+                   "parallelism": "1 GPU"},
         "roofline": roof,
-        "phases": {"element_kernel_ms": 1e3 * t_kernel, "scatter_ms": 1e3 * t_scatter,
-                   "scatter_GBs": scatter_gbs, "element_kernel_TFLOPs": kernel_tf},
+        "kernels": kernels,
+        "e2e": {"bytes": b_e2e, "GBs": b_e2e / step / 1e9,
+                "frac": b_e2e / step / 1e9 / HBM_PEAK_GBS},
     }
     out["fp32_form"] = fp32_form(prob, loc, blk, args, stream)
+    if not args.no_serving:
+        from hippymfem_amd import serving
+
+        out["serving"] = serving.bench_sample(seconds=20.0)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n)
     print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":
-    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    sys.path.insert(0, HERE)
     main()

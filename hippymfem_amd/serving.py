@@ -1,0 +1,357 @@
+"""The serving form of the assembly path: assemble-and-apply requests with continuous
+batching on one GPU.
+
+A request names a mesh (``n^3`` hexahedra of the unit cube), an order (P1 or P2) and a seed;
+the server draws the seeded parameter field ``m`` and direction ``x`` on the device,
+assembles the forward Jacobian ``A`` of ``exp(m) grad u . grad p - f p`` (Dirichlet x = 0, 1,
+folded) in the precision form (FP32 contraction, FP32 values) and returns ``y = A x`` to the
+host.  The reference assembles one problem per call (``benchmarks/bench_assembly.py``); this
+is the request loop around that call:
+
+* meshes, patterns, elimination plans and the CSR index arrays of every (n, order) are built
+  once and stay resident (``ModelCache``);
+* each in-flight request owns a slot: a HIP stream of default priority and a set of device
+  buffers (element matrices, values, m, x, y) taken from its model's pool;
+* requests are admitted in arrival order as soon as a slot is free and their buffers fit
+  the HBM budget beside those in flight; admitted requests run concurrently on their
+  streams, so their kernels overlap on the device, and a slot frees as soon as its request's
+  completion event fires;
+* latency is taken from a request's arrival to the host seeing its result.
+
+Every kernel of a request is deterministic (no float atomics), so a request's result does not
+depend on what else is in flight: the same bits alone and in the mix.
+"""
+
+import collections
+import time
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import ADJOINT, STATE
+from .problem import PDEVariationalProblem
+from .workloads import Workload
+
+SIZES = (8, 16, 24, 32, 48, 64)
+
+
+class Request:
+    """One assemble-and-apply request; ``arrival`` in seconds from the start of the stream."""
+
+    __slots__ = ("rid", "n", "order", "seed", "arrival", "admitted", "done", "y", "slot",
+                 "bufs", "event", "model")
+
+    def __init__(self, rid, n, order, seed, arrival=0.0):
+        self.rid, self.n, self.order, self.seed = int(rid), int(n), int(order), int(seed)
+        self.arrival = float(arrival)
+        self.admitted = self.done = None
+        self.y = self.slot = self.bufs = self.event = self.model = None
+
+    @property
+    def latency(self):
+        return None if self.done is None else self.done - self.arrival
+
+
+def poisson_requests(count, rate, seed=0, sizes=SIZES):
+    """``count`` requests arriving as a Poisson process of ``rate`` per second: n uniform
+    over ``sizes``, order P1 or P2 with probability 0.5, one seed each."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, count))
+    ns = rng.choice(np.asarray(sizes), count)
+    orders = rng.integers(1, 3, count)
+    seeds = rng.integers(0, 2 ** 31 - 1, count)
+    return [Request(k, ns[k], orders[k], seeds[k], t[k]) for k in range(count)]
+
+
+def device_field(n, seed, device, passes=3):
+    """The seeded parameter field on the ``(n+1)^3`` vertex lattice, drawn on the device:
+    N(0, 1) noise, ``passes`` of the 7-point average over the neighbours that exist, then
+    mean 0 and std 1 (the field of the benchmarks, drawn by the device generator)."""
+    g = torch.Generator(device=device)
+    g.manual_seed(int(seed))
+    m1 = n + 1
+    f = torch.randn((m1, m1, m1), generator=g, device=device, dtype=torch.float64)
+    cnt = torch.ones_like(f)
+    for ax in range(3):
+        cnt.narrow(ax, 1, m1 - 1).add_(1.0)
+        cnt.narrow(ax, 0, m1 - 1).add_(1.0)
+    for _ in range(passes):
+        acc = f.clone()
+        for ax in range(3):
+            acc.narrow(ax, 1, m1 - 1).add_(f.narrow(ax, 0, m1 - 1))
+            acc.narrow(ax, 0, m1 - 1).add_(f.narrow(ax, 1, m1 - 1))
+        f = acc / cnt
+    f = f.reshape(-1)
+    return (f - f.mean()) / f.std()
+
+
+def device_direction(ndof, seed, device):
+    g = torch.Generator(device=device)
+    g.manual_seed(int(seed) + 1)
+    return torch.randn(ndof, generator=g, device=device, dtype=torch.float64)
+
+
+class Buffers:
+    """The device memory of one in-flight request of a model."""
+
+    def __init__(self, model):
+        dev = model.device
+        k = model.prob.kernel
+        e_dtype = torch.float64 if model.precision == "fp64" else torch.float32
+        self.E = torch.empty((k.group.ne, k.nd[ADJOINT], k.nd[STATE]), dtype=e_dtype, device=dev)
+        self.data = torch.empty(model.pattern.nnz, dtype=e_dtype, device=dev)
+        self.y = torch.empty(model.ndof, dtype=torch.float64, device=dev)
+        self.y_host = torch.empty(model.ndof, dtype=torch.float64, pin_memory=True)
+        self.nbytes = (self.E.numel() * self.E.element_size() + self.data.numel() *
+                       self.data.element_size() + 8 * (2 * model.ndof + model.ndof_m))
+
+
+class Model:
+    """Everything of one (n, order) that stays resident: the mesh and space tables, the
+    pattern and its elimination plan, the CSR index arrays, and a pool of request buffers."""
+
+    def __init__(self, n, order, precision, device):
+        self.n, self.order, self.precision, self.device = n, order, precision, device
+        W = Workload(n, order=order)
+        self.W = W
+        self.prob = PDEVariationalProblem(W.spaces, W.density, W.batches, W.bc)
+        blk = self.prob.block_assembly(ADJOINT, STATE, precision=precision)
+        self.plan = blk.plan
+        self.dplan = blk.dplan
+        self.pattern = blk.pattern
+        self.blk = blk
+        self.ndof = W.Vu.ndof
+        self.ndof_m = W.Vm.ndof
+        # CSR index arrays on the device (the matrix-vector product reads them)
+        self.indptr = torch.as_tensor(self.pattern.indptr).to(device)
+        self.indices = torch.as_tensor(self.pattern.indices).to(device)
+        # lattice order -> dof order of the parameter space, and the fields A does not read
+        self.lattice_to_dof_m = torch.as_tensor(W.Vm.lattice_to_dof.astype(np.int64)).to(device)
+        self.zeros_u = torch.zeros(self.ndof, dtype=torch.float64, device=device)
+        self.pool = []
+        k = self.prob.kernel
+        esz = 8 if precision == "fp64" else 4
+        self.request_bytes = (esz * k.group.ne * k.nd[ADJOINT] * k.nd[STATE] + esz *
+                              self.pattern.nnz + 8 * (2 * self.ndof + self.ndof_m))
+
+    def take(self):
+        return self.pool.pop() if self.pool else Buffers(self)
+
+    def give(self, b):
+        self.pool.append(b)
+
+    def launch(self, req, bufs, stream):
+        """The request's work on its stream: draw m and x, element matrices, scatter,
+        y = A x, y to the host; returns the completion event."""
+        with torch.cuda.stream(stream):
+            lat = device_field(self.n, req.seed, self.device)
+            m = torch.empty_like(lat)
+            m[self.lattice_to_dof_m] = lat
+            x = device_direction(self.ndof, req.seed, self.device)
+            loc = [self.zeros_u, m, self.zeros_u]
+            self.prob.kernel.element_matrices(ADJOINT, STATE, loc, precision=self.precision,
+                                              out=bufs.E, stream=stream)
+            self.pattern.data([bufs.E], plan=self.dplan, out=bufs.data, stream=stream)
+            code = _lib.FP64 if bufs.data.dtype == torch.float64 else _lib.FP32
+            _lib.check(_lib.lib().hm_csr_matvec(self.ndof, _lib.ptr(self.indptr),
+                                                _lib.ptr(self.indices), _lib.ptr(bufs.data),
+                                                code, _lib.ptr(x), _lib.ptr(bufs.y),
+                                                _lib.stream_handle(stream)), "serving matvec")
+            bufs.y_host.copy_(bufs.y, non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(stream)
+            # the request's temporaries stay alive until the event fires
+            bufs.keep = (lat, m, x)
+        return ev
+
+
+class ModelCache:
+    def __init__(self, precision="fp32", device=None):
+        self.precision = precision
+        self.device = device or _lib.require_device()
+        self.models = {}
+
+    def get(self, n, order):
+        key = (int(n), int(order))
+        m = self.models.get(key)
+        if m is None:
+            m = self.models[key] = Model(key[0], key[1], self.precision, self.device)
+        return m
+
+    def warm(self, sizes=SIZES, orders=(1, 2)):
+        for n in sizes:
+            for o in orders:
+                self.get(n, o)
+
+
+class ServingEngine:
+    """Continuous batching over ``slots`` streams within ``hbm_budget`` bytes of request
+    buffers."""
+
+    def __init__(self, cache, slots=8, hbm_budget=48 << 30):
+        self.cache = cache
+        self.slots = [torch.cuda.Stream(device=cache.device) for _ in range(int(slots))]
+        self.budget = int(hbm_budget)
+        self.peak_bytes = 0
+        self.peak_inflight = 0
+
+    def run(self, requests, keep_results=True, poll_sleep=20e-6):
+        """Serve ``requests`` (sorted by arrival); returns them with ``admitted``, ``done``
+        and (when ``keep_results``) ``y`` set.  Times are seconds from the start."""
+        for r in requests:
+            if self.cache.get(r.n, r.order).request_bytes > self.budget:
+                raise ValueError("request %d needs more than the HBM budget" % r.rid)
+        pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
+        waiting = collections.deque()
+        free = list(range(len(self.slots)))
+        inflight = []
+        used = 0
+        t0 = time.perf_counter()
+        while pending or waiting or inflight:
+            now = time.perf_counter() - t0
+            while pending and pending[0].arrival <= now:
+                waiting.append(pending.popleft())
+            # admission: arrival order, a free slot and room in the budget
+            while waiting and free:
+                r = waiting[0]
+                model = self.cache.get(r.n, r.order)
+                if used + model.request_bytes > self.budget:
+                    break
+                waiting.popleft()
+                r.model = model
+                r.slot = free.pop()
+                r.bufs = model.take()
+                used += model.request_bytes
+                r.admitted = time.perf_counter() - t0
+                r.event = model.launch(r, r.bufs, self.slots[r.slot])
+                inflight.append(r)
+                self.peak_bytes = max(self.peak_bytes, used)
+                self.peak_inflight = max(self.peak_inflight, len(inflight))
+            # completions
+            still = []
+            for r in inflight:
+                if r.event.query():
+                    r.done = time.perf_counter() - t0
+                    if keep_results:
+                        r.y = r.bufs.y_host.clone()
+                    r.bufs.keep = None
+                    r.model.give(r.bufs)
+                    used -= r.model.request_bytes
+                    free.append(r.slot)
+                    r.bufs = r.event = None
+                else:
+                    still.append(r)
+            inflight = still
+            if not waiting and inflight:
+                time.sleep(poll_sleep)
+            elif not inflight and not waiting and pending:
+                time.sleep(min(poll_sleep, max(0.0, pending[0].arrival - now)))
+        return requests
+
+
+def reference_product(cache, req, precision="fp64"):
+    """``y = A x`` of a request in FP64 (or ``precision``), alone on the current stream."""
+    model = cache.get(req.n, req.order)
+    if precision != model.precision:
+        refs = cache.__dict__.setdefault("refs", {})
+        key = (precision, req.n, req.order)
+        if key not in refs:
+            refs[key] = Model(req.n, req.order, precision, cache.device)
+        model = refs[key]
+    bufs = model.take()
+    stream = torch.cuda.current_stream()
+    ev = model.launch(req, bufs, stream)
+    ev.synchronize()
+    y = bufs.y_host.clone()
+    bufs.keep = None
+    model.give(bufs)
+    return y
+
+
+def summary(requests, wall=None):
+    lat = np.array([r.latency for r in requests if r.latency is not None])
+    done = [r for r in requests if r.done is not None]
+    span = wall if wall is not None else (max(r.done for r in done) - min(r.arrival for r in done))
+    nnz = sum(r.model.pattern.nnz for r in done if r.model is not None)
+    return {"requests": len(done), "throughput_req_s": len(done) / span,
+            "throughput_nnz_s": nnz / span,
+            "p50_ms": 1e3 * float(np.percentile(lat, 50)), "p99_ms": 1e3 * float(np.percentile(lat, 99)),
+            "mean_ms": 1e3 * float(lat.mean()), "max_ms": 1e3 * float(lat.max())}
+
+
+def error_vs_fp64(cache, requests, count=8):
+    """Largest relative error (2-norm) of served products against FP64 ones, over the first
+    ``count`` requests of distinct (n, order)."""
+    seen = set()
+    worst = 0.0
+    for r in requests:
+        key = (r.n, r.order)
+        if key in seen or r.y is None:
+            continue
+        seen.add(key)
+        y64 = reference_product(cache, r, "fp64")
+        worst = max(worst, float(torch.linalg.vector_norm(r.y - y64) / torch.linalg.vector_norm(y64)))
+        if len(seen) >= count:
+            break
+    return worst
+
+
+def bench_sample(seconds=20.0, rate=None, slots=8, seed=0):
+    """A bounded serving run for the benchmark line: requests at ``rate`` per second (by
+    default about half the measured capacity) for about ``seconds``; returns latency and
+    throughput figures and the error against FP64."""
+    cache = ModelCache("fp32")
+    cache.warm()
+    eng = ServingEngine(cache, slots=slots)
+    # capacity: a closed burst of mixed requests
+    burst = poisson_requests(48, 1e6, seed=seed + 1)
+    for r in burst:
+        r.arrival = 0.0
+    t0 = time.perf_counter()
+    eng.run(burst, keep_results=False)
+    cap = len(burst) / (time.perf_counter() - t0)
+    rate = rate or 0.5 * cap
+    count = max(32, min(2000, int(rate * seconds)))
+    reqs = poisson_requests(count, rate, seed=seed)
+    eng.run(reqs)
+    out = summary(reqs)
+    out.update({"rate_req_s": rate, "capacity_req_s": cap, "slots": slots,
+                "precision": "fp32 contraction and values",
+                "max_rel_err_vs_fp64": error_vs_fp64(cache, reqs)})
+    return out
+
+
+def main(argv=None):
+    """``python -m hippymfem_amd.serving --requests 2000 --rates 0.25,0.5,0.9``: rates as
+    fractions of the measured capacity (or absolute with ``--absolute``)."""
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--requests", type=int, default=2000)
+    ap.add_argument("--rates", default="0.25,0.5,0.9")
+    ap.add_argument("--absolute", action="store_true")
+    ap.add_argument("--slots", type=int, default=8)
+    a = ap.parse_args(argv)
+    cache = ModelCache("fp32")
+    cache.warm()
+    eng = ServingEngine(cache, slots=a.slots)
+    burst = poisson_requests(96, 1e6, seed=7)
+    for r in burst:
+        r.arrival = 0.0
+    t0 = time.perf_counter()
+    eng.run(burst, keep_results=False)
+    cap = len(burst) / (time.perf_counter() - t0)
+    print(json.dumps({"capacity_req_s": cap}), flush=True)
+    for k, f in enumerate(float(x) for x in a.rates.split(",")):
+        rate = f if a.absolute else f * cap
+        reqs = poisson_requests(a.requests, rate, seed=100 + k)
+        eng.run(reqs)
+        out = summary(reqs)
+        out.update({"rate_req_s": rate, "max_rel_err_vs_fp64": error_vs_fp64(cache, reqs)})
+        print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

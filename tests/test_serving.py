@@ -1,0 +1,89 @@
+"""The serving form: continuous batching of assemble-and-apply requests.
+
+A served product is checked against the oracle (FP64), the precision form against FP64
+(relative 2-norm error at most 1e-6), every result of a mixed stream against the same request
+served alone (bit-identical: the kernels are deterministic), and admission against its HBM
+budget.
+"""
+
+import numpy as np
+import pytest
+
+from hippymfem_amd.serving import SIZES, poisson_requests
+
+
+def test_poisson_requests():
+    reqs = poisson_requests(2000, 50.0, seed=3)
+    t = np.array([r.arrival for r in reqs])
+    assert len(reqs) == 2000 and np.all(np.diff(t) >= 0)
+    assert abs(np.diff(t).mean() * 50.0 - 1.0) < 0.1
+    assert {r.n for r in reqs} <= set(SIZES) and {r.order for r in reqs} == {1, 2}
+    assert abs(np.mean([r.order == 2 for r in reqs]) - 0.5) < 0.05
+    again = poisson_requests(2000, 50.0, seed=3)
+    assert [(r.n, r.order, r.seed) for r in again] == [(r.n, r.order, r.seed) for r in reqs]
+
+
+@pytest.mark.gpu
+def test_served_product_matches_oracle(gpu):
+    from oracle import fe_oracle as O
+
+    from hippymfem_amd import ADJOINT, STATE
+    from hippymfem_amd.serving import (ModelCache, Request, ServingEngine, device_direction,
+                                       device_field)
+
+    cache = ModelCache("fp64")
+    eng = ServingEngine(cache, slots=2)
+    for order in (1, 2):
+        r = Request(0, 4, order, seed=123 + order)
+        eng.run([r])
+        P = O.Problem(4, order=order)
+        P.m = P.Vm.lattice_field(device_field(4, r.seed, gpu).cpu().numpy())
+        x = device_direction(P.Vu.ndof, r.seed, gpu).cpu().numpy()
+        ip, ix, d = P.assembled(ADJOINT, STATE)
+        y = O.csr_matvec(ip, ix, d, x)
+        err = np.abs(r.y.numpy() - y).max() / np.abs(y).max()
+        print("order %d served vs oracle %.2e" % (order, err), flush=True)
+        assert err <= 1e-13
+
+
+@pytest.mark.gpu
+def test_alone_equals_in_the_mix(gpu):
+    import torch
+
+    from hippymfem_amd.serving import ModelCache, Request, ServingEngine
+
+    cache = ModelCache("fp32")
+    eng = ServingEngine(cache, slots=4)
+    spec = [(8, 1, 11), (8, 2, 12), (16, 1, 13), (16, 2, 14), (24, 2, 15), (8, 2, 16),
+            (16, 2, 17), (4, 1, 18)]
+    mix = [Request(k, n, o, s, arrival=0.0005 * k) for k, (n, o, s) in enumerate(spec)]
+    eng.run(mix)
+    assert eng.peak_inflight > 1                      # the requests did overlap
+    for r in mix:
+        alone = Request(100 + r.rid, r.n, r.order, r.seed)
+        ServingEngine(cache, slots=1).run([alone])
+        assert torch.equal(alone.y, r.y), "request %d differs alone and in the mix" % r.rid
+
+
+@pytest.mark.gpu
+def test_precision_form_error_and_admission(gpu):
+    import torch
+
+    from hippymfem_amd.serving import ModelCache, ServingEngine, reference_product
+
+    cache = ModelCache("fp32")
+    reqs = poisson_requests(12, 2000.0, seed=5, sizes=(8, 16, 24))
+    largest = max(cache.get(r.n, r.order).request_bytes for r in reqs)
+    eng = ServingEngine(cache, slots=8, hbm_budget=int(1.5 * largest))
+    eng.run(reqs)
+    assert all(r.y is not None for r in reqs)
+    assert eng.peak_bytes <= int(1.5 * largest)
+    worst = 0.0
+    for r in reqs:
+        y64 = reference_product(cache, r, "fp64")
+        worst = max(worst, float(torch.linalg.vector_norm(r.y - y64) /
+                                 torch.linalg.vector_norm(y64)))
+    print("fp32 served vs fp64: %.2e" % worst, flush=True)
+    assert worst <= 1e-6
+    with pytest.raises(ValueError):
+        ServingEngine(cache, slots=2, hbm_budget=1024).run(reqs[:1])
