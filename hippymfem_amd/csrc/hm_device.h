@@ -26,8 +26,5 @@ int hm_element_matrices_mfma(const hm_problem* P, int i, int j, int64_t e0, int6
 // HM_ERR_UNSUPPORTED when the problem has no tensor structure.
 int hm_element_matrices_sumfact(const hm_problem* P, int i, int j, int64_t e0, int64_t e1,
                                 int precision, void* out, void* stream);
-// BF16 / FP16 products with FP32 accumulation (hm_precision.hip).
-int hm_element_matrices_lowp(const hm_problem* P, int i, int j, int64_t e0, int64_t e1,
-                             int precision, void* out, void* stream);
 // Fills n words of 4 or 8 bytes with zero on the stream (a kernel: safe under capture).
 int hm_fill_zero(void* p, size_t n, int word_bytes, hipStream_t st);

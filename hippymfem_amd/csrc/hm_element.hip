@@ -224,8 +224,9 @@ extern "C" int hm_element_matrices(const hm_problem* P, int i, int j, int64_t e0
   const char* route = getenv("HM_CONTRACTION");
   const bool valu = route && route[0] == 'v';
   const bool mfma = route && route[0] == 'm';
+  // BF16 / FP16 operands: the sum-factorized contraction only
   if (precision == HM_BF16 || precision == HM_FP16)
-    return hm_element_matrices_lowp(P, i, j, e0, e1, precision, out, stream);
+    return hm_element_matrices_sumfact(P, i, j, e0, e1, precision, out, stream);
   HM_REQUIRE(precision == HM_FP64 || precision == HM_FP32, "unknown precision %d", precision);
   if (!valu && !mfma) {
     const int rc = hm_element_matrices_sumfact(P, i, j, e0, e1, precision, out, stream);

@@ -16,6 +16,8 @@
 // contraction tasks: each runs stages 1 and 2 for its (a1, b1, a2) in registers and stage 3
 // into its NC NT NC entries (b2, a3, b3) of the element matrix, which it writes to memory.
 // The pointwise pairs never leave the compute unit.
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include "hm_common.h"
@@ -42,6 +44,17 @@ struct SumfactPlan {
 
 constexpr int SF_WPB = 4;  // wavefronts a block
 
+// The operand format of the contraction: QP = HM_FP64 or HM_FP32 leave operands as they are;
+// HM_BF16 / HM_FP16 round every operand of a product (the scaled pointwise pairs, the 1-D
+// table products and the stage results fed to the next stage) to that format, while the
+// products and sums stay FP32 (a product of two such operands is exact in FP32).
+template <int QP, typename T>
+__device__ inline T quant(T x) {
+  if constexpr (QP == HM_BF16) return __bfloat162float(__float2bfloat16((float)x));
+  else if constexpr (QP == HM_FP16) return __half2float(__float2half((float)x));
+  else return x;
+}
+
 // The lanes of a wavefront see each other's LDS writes: a wavefront's LDS operations complete
 // in order, so a fence against compiler reordering and a wavefront barrier suffice.
 __device__ inline void wave_sync() {
@@ -63,7 +76,7 @@ __host__ __device__ inline int sf_table_words(int nd_u, int nd_m, bool fields) {
   return 27 * 24 + 27 * nd_m + (fields ? 27 * nd_u * 4 : 0);
 }
 
-template <typename CT, int NT, int NC>
+template <typename CT, int NT, int NC, int QP>
 __global__ void __launch_bounds__(256) k_sumfact(const hm_problem P, int s, int j, int64_t e0, int64_t e1,
                                                  SumfactPlan SP, const double* __restrict__ t1t,
                                                  const double* __restrict__ t1c, CT* __restrict__ out) {
@@ -195,7 +208,10 @@ __global__ void __launch_bounds__(256) k_sumfact(const hm_problem P, int s, int 
     const int sh = in_use ? slot * LPE : 0;
     const uint64_t both = ((bal0 >> sh) & lpe_mask) | (((bal1 >> sh) & lpe_mask) << LPE);
     const uint32_t live = (uint32_t)both & ((1u << npair) - 1u);
-    // 4. lane = contraction task (a1, b1, a2)
+    // 4. lane = contraction task (a1, b1, a2).  The low-precision forms scale the element's
+    // pairs by its largest pointwise weight (operands within [-1, 1]) and the result back.
+    constexpr bool LOWP = QP == HM_BF16 || QP == HM_FP16;
+    const double wscale = (LOWP && top > 0.0) ? 1.0 / top : 1.0;
     CT acc[NC][NT][NC];
 #pragma unroll
     for (int b2 = 0; b2 < NC; ++b2)
@@ -230,7 +246,7 @@ __global__ void __launch_bounds__(256) k_sumfact(const hm_problem P, int s, int 
           hd[b2][2] = f2[2];
         }
         __builtin_amdgcn_sched_barrier(0);
-        const CT g0 = (CT)g0d, g1 = (CT)g1d, g2 = (CT)g2d;
+        const CT g0 = quant<QP>((CT)g0d), g1 = quant<QP>((CT)g1d), g2 = quant<QP>((CT)g2d);
         // one q3 plane at a time:
         //   stage 1: s1[q2] = sum_q1 X(a1, q1) Y(b1, q1) W(q1, q2, q3)
         //   stage 2: s2[b2][q3] += sum_q2 X(a2, q2) Y(b2, q2) s1[q2]
@@ -240,11 +256,18 @@ __global__ void __launch_bounds__(256) k_sumfact(const hm_problem P, int s, int 
 #pragma unroll
           for (int q2 = 0; q2 < 3; ++q2) {
             const double* wr = wv + 3 * q2 + 9 * q3;
-            s1[q2] = fma(g2, (CT)wr[2], fma(g1, (CT)wr[1], g0 * (CT)wr[0]));
+            if constexpr (LOWP) {
+              const CT w0 = quant<QP>((CT)(wr[0] * wscale)), w1 = quant<QP>((CT)(wr[1] * wscale)),
+                       w2 = quant<QP>((CT)(wr[2] * wscale));
+              s1[q2] = quant<QP>(fma(g2, w2, fma(g1, w1, g0 * w0)));
+            } else {
+              s1[q2] = fma(g2, (CT)wr[2], fma(g1, (CT)wr[1], g0 * (CT)wr[0]));
+            }
           }
 #pragma unroll
           for (int b2 = 0; b2 < NC; ++b2)
-            s2[b2][q3] = fma((CT)hd[b2][2], s1[2], fma((CT)hd[b2][1], s1[1], fma((CT)hd[b2][0], s1[0], s2[b2][q3])));
+            s2[b2][q3] = fma(quant<QP>((CT)hd[b2][2]), s1[2],
+                             fma(quant<QP>((CT)hd[b2][1]), s1[1], fma(quant<QP>((CT)hd[b2][0]), s1[0], s2[b2][q3])));
         }
       }
       // stage 3 for this type of third-direction factors, the 1-D rows read as wavefront-uniform
@@ -252,6 +275,24 @@ __global__ void __launch_bounds__(256) k_sumfact(const hm_problem P, int s, int 
       //   E[a3][b3] += sum_q3 Xt(a3, q3) (Yc(b3, q3) s2[b2][q3])
       const double* bt = t1t + (t >> 1) * NT * 3;
       const double* bc = t1c + (t & 1) * NC * 3;
+      if constexpr (LOWP) {
+        // the stage-3 operands: s2 and the table products, each rounded
+#pragma unroll
+        for (int b2 = 0; b2 < NC; ++b2)
+#pragma unroll
+          for (int q = 0; q < 3; ++q) s2[b2][q] = quant<QP>(s2[b2][q]);
+#pragma unroll
+        for (int a3 = 0; a3 < NT; ++a3)
+#pragma unroll
+          for (int b3 = 0; b3 < NC; ++b3) {
+            const double* f3 = PT + ((t * NT + a3) * NC + b3) * 3;
+            const CT k0 = quant<QP>((CT)f3[0]), k1 = quant<QP>((CT)f3[1]), k2 = quant<QP>((CT)f3[2]);
+#pragma unroll
+            for (int b2 = 0; b2 < NC; ++b2)
+              acc[b2][a3][b3] = fma(k2, s2[b2][2], fma(k1, s2[b2][1], fma(k0, s2[b2][0], acc[b2][a3][b3])));
+          }
+        continue;
+      }
 #pragma unroll
       for (int b3 = 0; b3 < NC; ++b3) {
         CT u[NC][3];
@@ -276,7 +317,8 @@ __global__ void __launch_bounds__(256) k_sumfact(const hm_problem P, int s, int 
 #pragma unroll
         for (int b2 = 0; b2 < NC; ++b2)
 #pragma unroll
-          for (int b3 = 0; b3 < NC; ++b3) Er[lc[b1 + NC * (b2 + NC * b3)]] = acc[b2][a3][b3];
+          for (int b3 = 0; b3 < NC; ++b3)
+            Er[lc[b1 + NC * (b2 + NC * b3)]] = LOWP ? (CT)(acc[b2][a3][b3] * top) : acc[b2][a3][b3];
       }
     }
     // the LDS of this element is rewritten by the next
@@ -284,7 +326,7 @@ __global__ void __launch_bounds__(256) k_sumfact(const hm_problem P, int s, int 
   }
 }
 
-template <typename CT, int NT, int NC>
+template <typename CT, int NT, int NC, int QP>
 static int launch_sumfact(const hm_problem& P, int i, int j, int64_t e0, int64_t e1, const SumfactPlan& SP,
                           void* out, hipStream_t st) {
   constexpr int EPW = 64 / (NT * NC * NT);
@@ -298,14 +340,14 @@ static int launch_sumfact(const hm_problem& P, int i, int j, int64_t e0, int64_t
   if (bytes + 4 * NT * NC * 3 * sizeof(double) > (size_t)limit)
     HM_FAIL(HM_ERR_UNSUPPORTED, "the sum-factorized contraction needs %zu bytes of LDS", bytes);
   if (bytes > 48 * 1024) {
-    hipError_t err = hipFuncSetAttribute((const void*)k_sumfact<CT, NT, NC>,
+    hipError_t err = hipFuncSetAttribute((const void*)k_sumfact<CT, NT, NC, QP>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (err != hipSuccess) HM_FAIL(HM_ERR_HIP, "hipFuncSetAttribute: %s", hipGetErrorString(err));
   }
   // persistent: as many blocks as the device holds at once, each walking many elements (the
   // tables are loaded once a block and the next element's data is in flight meanwhile)
   int per_cu = 0, cus = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_sumfact<CT, NT, NC>, 256, bytes) !=
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_sumfact<CT, NT, NC, QP>, 256, bytes) !=
           hipSuccess ||
       per_cu < 1)
     per_cu = 1;
@@ -313,10 +355,19 @@ static int launch_sumfact(const hm_problem& P, int i, int j, int64_t e0, int64_t
   const int64_t groups = (e1 - e0 + SF_WPB * EPW - 1) / (SF_WPB * EPW);
   const int64_t full = (int64_t)per_cu * cus;
   const int grid = (int)(groups < full ? groups : full);
-  hipLaunchKernelGGL((k_sumfact<CT, NT, NC>), dim3(grid), dim3(256), bytes, st, P, i, j, e0, e1, SP,
+  hipLaunchKernelGGL((k_sumfact<CT, NT, NC, QP>), dim3(grid), dim3(256), bytes, st, P, i, j, e0, e1, SP,
                      (const double*)(i == HM_PARAMETER ? P.tab1_m : P.tab1_u),
                      (const double*)(j == HM_PARAMETER ? P.tab1_m : P.tab1_u), (CT*)out);
   return hip_check("k_sumfact");
+}
+
+template <typename CT, int QP>
+static int dispatch_orders(const hm_problem& P, int i, int j, int64_t e0, int64_t e1, int nt, int nc,
+                           const SumfactPlan& SP, void* out, hipStream_t st) {
+  if (nt == 3 && nc == 3) return launch_sumfact<CT, 3, 3, QP>(P, i, j, e0, e1, SP, out, st);
+  if (nt == 3 && nc == 2) return launch_sumfact<CT, 3, 2, QP>(P, i, j, e0, e1, SP, out, st);
+  if (nt == 2 && nc == 3) return launch_sumfact<CT, 2, 3, QP>(P, i, j, e0, e1, SP, out, st);
+  return launch_sumfact<CT, 2, 2, QP>(P, i, j, e0, e1, SP, out, st);
 }
 
 }  // namespace hm
@@ -370,15 +421,11 @@ int hm_element_matrices_sumfact(const hm_problem* P, int i, int j, int64_t e0, i
   for (int p = 0; p < np; ++p) SP.tmask[((SP.pr[p] == 3) << 1) | (SP.pc[p] == 3)] |= 1 << p;
   SP.need_fields = !(i * 3 + j == 6 && !nl);  // the linear Jacobian reads m and the geometry only
   hipStream_t st = (hipStream_t)stream;
-  if (precision == HM_FP64) {
-    if (nt == 3 && nc == 3) return launch_sumfact<double, 3, 3>(*P, i, j, e0, e1, SP, out, st);
-    if (nt == 3 && nc == 2) return launch_sumfact<double, 3, 2>(*P, i, j, e0, e1, SP, out, st);
-    if (nt == 2 && nc == 3) return launch_sumfact<double, 2, 3>(*P, i, j, e0, e1, SP, out, st);
-    return launch_sumfact<double, 2, 2>(*P, i, j, e0, e1, SP, out, st);
+  switch (precision) {
+    case HM_FP64: return dispatch_orders<double, HM_FP64>(*P, i, j, e0, e1, nt, nc, SP, out, st);
+    case HM_FP32: return dispatch_orders<float, HM_FP32>(*P, i, j, e0, e1, nt, nc, SP, out, st);
+    case HM_BF16: return dispatch_orders<float, HM_BF16>(*P, i, j, e0, e1, nt, nc, SP, out, st);
+    case HM_FP16: return dispatch_orders<float, HM_FP16>(*P, i, j, e0, e1, nt, nc, SP, out, st);
+    default: HM_FAIL(HM_ERR_UNSUPPORTED, "precision %d", precision);
   }
-  if (precision != HM_FP32) HM_FAIL(HM_ERR_UNSUPPORTED, "precision %d", precision);
-  if (nt == 3 && nc == 3) return launch_sumfact<float, 3, 3>(*P, i, j, e0, e1, SP, out, st);
-  if (nt == 3 && nc == 2) return launch_sumfact<float, 3, 2>(*P, i, j, e0, e1, SP, out, st);
-  if (nt == 2 && nc == 3) return launch_sumfact<float, 2, 3>(*P, i, j, e0, e1, SP, out, st);
-  return launch_sumfact<float, 2, 2>(*P, i, j, e0, e1, SP, out, st);
 }

@@ -253,3 +253,29 @@ def test_fp32_contraction_error(gpu, nonlinear, route, monkeypatch):
         ipo, ixo, do = P.assembled(i, j)
         assert np.array_equal(ip, ipo) and np.array_equal(ix, ixo)
         assert rel(d, do) <= 1e-6, (i, j, rel(d, do))
+
+
+# Error bounds of the low-precision forms against the FP64 oracle, relative to the largest
+# entry: operands rounded to 8 (BF16) or 11 (FP16) significant bits, about three roundings
+# along each product chain, FP32 sums.
+LOWP_BOUND = {"bf16": 2e-2, "fp16": 3e-3}
+
+
+@pytest.mark.parametrize("precision", ["bf16", "fp16"])
+def test_lowp_contraction_error(gpu, precision):
+    """BF16 / FP16 operands with FP32 accumulation and per-element scaling: every block of
+    a perturbed mesh within the bound of the FP64 oracle; the error is reported."""
+    W, P = pair(6, perturb=0.2, nonlinear=True)
+    for i, j in BLOCKS + [(STATE, STATE)]:
+        E = W.kernel.element_matrices(i, j, W.locals(), precision=precision)[0]
+        assert E.dtype == torch.float32
+        err = rel(E.cpu().numpy(), P.block(i, j))
+        print("%s block (%d, %d): %.2e" % (precision, i, j, err), flush=True)
+        assert err <= LOWP_BOUND[precision], (i, j, err)
+        assert err > 0.0                          # the operands were rounded
+
+
+def test_lowp_needs_tensor_structure(gpu):
+    W = Workload(2, quadrature_degree=6)            # 4 points a direction: no sum factorization
+    with pytest.raises(NotImplementedError):
+        W.kernel.element_matrices(ADJOINT, STATE, W.locals(), precision="bf16")
