@@ -1,0 +1,107 @@
+"""The assembly configurations beside the headline one, each printed as JSON lines.
+
+* ``precision``: the precision form on a perturbed mesh (interior vertices moved by up to
+  0.2 h, so det J varies): every block assembled in FP64 and in the FP32, BF16 and FP16
+  contraction forms; the error of each form against the FP64 values (largest entry error
+  over the largest entry, and relative Frobenius norm) and the time of each assembly.
+* ``nonlinear``: ``exp(m) (1 + u^2) grad u . grad p - f p`` at a smooth state (8 Fourier
+  modes, amplitude 0.5): the residual and every block in FP64 and FP32, times and errors.
+
+``python -m hippymfem_amd.configs precision --n 128`` / ``nonlinear --n 96``.
+"""
+
+import argparse
+import json
+import time
+
+import numpy as np
+import torch
+
+from ._lib import ADJOINT, PARAMETER, STATE
+from .problem import PDEVariationalProblem
+from .workloads import Workload
+
+NAMES = {(ADJOINT, STATE): "A", (ADJOINT, PARAMETER): "C", (STATE, PARAMETER): "W_um",
+         (PARAMETER, PARAMETER): "W_mm", (STATE, STATE): "W_uu"}
+
+
+def _time(fn, steps):
+    fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def _errors(d, ref):
+    d = d.double()
+    diff = d - ref
+    return {"max_rel": float(diff.abs().max() / ref.abs().max()),
+            "fro_rel": float(torch.linalg.vector_norm(diff) / torch.linalg.vector_norm(ref))}
+
+
+def _blocks(prob, loc, blocks, precisions, steps, n):
+    out = []
+    for ij in blocks:
+        ref_blk = prob.block_assembly(*ij, precision="fp64")
+        t64 = _time(lambda: ref_blk.assemble(loc), steps)
+        ref = ref_blk.data.clone()
+        nnz = ref_blk.pattern.nnz
+        line = {"n": n, "block": NAMES[ij], "nnz": nnz, "fp64_ms": t64,
+                "fp64_nnz_s": nnz / (t64 * 1e-3)}
+        del prob._blocks[(ij[0], ij[1], "fp64", None)]
+        del ref_blk
+        for prec in precisions:
+            blk = prob.block_assembly(*ij, precision=prec)
+            t = _time(lambda: blk.assemble(loc), steps)
+            line[prec] = dict(ms=t, nnz_s=nnz / (t * 1e-3), speedup_vs_fp64=t64 / t,
+                              **_errors(blk.data, ref))
+            del prob._blocks[(ij[0], ij[1], prec, None)]
+            del blk
+        del ref
+        torch.cuda.empty_cache()
+        print(json.dumps(line), flush=True)
+        out.append(line)
+    return out
+
+
+def precision_form(n=128, perturb=0.2, steps=3):
+    W = Workload(n, perturb=perturb)
+    prob = PDEVariationalProblem(W.spaces, W.density, W.batches, W.bc)
+    loc = [torch.as_tensor(v).cuda() for v in W.locals()]
+    blocks = [(ADJOINT, STATE), (ADJOINT, PARAMETER), (STATE, PARAMETER), (PARAMETER, PARAMETER)]
+    return _blocks(prob, loc, blocks, ("fp32", "bf16", "fp16"), steps, n)
+
+
+def nonlinear(n=96, steps=3):
+    W = Workload(n, nonlinear=True, state="fourier")
+    prob = PDEVariationalProblem(W.spaces, W.density, W.batches, W.bc)
+    loc = [torch.as_tensor(v).cuda() for v in W.locals()]
+    x = [loc[0], loc[1], loc[2]]
+    t_res = _time(lambda: prob.residual(x), steps)
+    print(json.dumps({"n": n, "residual_fp64_ms": t_res, "dofs": W.Vu.ndof}), flush=True)
+    blocks = [(ADJOINT, STATE), (ADJOINT, PARAMETER), (STATE, STATE), (STATE, PARAMETER),
+              (PARAMETER, PARAMETER)]
+    return _blocks(prob, loc, blocks, ("fp32",), steps, n)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("which", choices=["precision", "nonlinear"])
+    ap.add_argument("--n", type=int, default=None)
+    ap.add_argument("--steps", type=int, default=3)
+    a = ap.parse_args(argv)
+    t0 = time.perf_counter()
+    if a.which == "precision":
+        precision_form(a.n or 128, steps=a.steps)
+    else:
+        nonlinear(a.n or 96, steps=a.steps)
+    print(json.dumps({"wall_s": time.perf_counter() - t0}), flush=True)
+
+
+if __name__ == "__main__":
+    main()

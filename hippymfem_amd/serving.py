@@ -323,14 +323,15 @@ def bench_sample(seconds=20.0, rate=None, slots=8, seed=0):
 
 
 def main(argv=None):
-    """``python -m hippymfem_amd.serving --requests 2000 --rates 0.25,0.5,0.9``: rates as
-    fractions of the measured capacity (or absolute with ``--absolute``)."""
+    """``python -m hippymfem_amd.serving --requests 2000 --rates 0.5,0.8,0.95``: Poisson
+    arrivals at fractions of the measured saturated throughput (or absolute rates with
+    ``--absolute``), arrival seeds 5, 6, 7."""
     import argparse
     import json
 
     ap = argparse.ArgumentParser()
     ap.add_argument("--requests", type=int, default=2000)
-    ap.add_argument("--rates", default="0.25,0.5,0.9")
+    ap.add_argument("--rates", default="0.5,0.8,0.95")
     ap.add_argument("--absolute", action="store_true")
     ap.add_argument("--slots", type=int, default=8)
     a = ap.parse_args(argv)
@@ -346,7 +347,7 @@ def main(argv=None):
     print(json.dumps({"capacity_req_s": cap}), flush=True)
     for k, f in enumerate(float(x) for x in a.rates.split(",")):
         rate = f if a.absolute else f * cap
-        reqs = poisson_requests(a.requests, rate, seed=100 + k)
+        reqs = poisson_requests(a.requests, rate, seed=5 + k)
         eng.run(reqs)
         out = summary(reqs)
         out.update({"rate_req_s": rate, "max_rel_err_vs_fp64": error_vs_fp64(cache, reqs)})
