@@ -71,16 +71,17 @@ __global__ void __launch_bounds__(256) k_scatter_rows(hm_scatter_plan S, const E
           c[k] = DROP;
           v[k] = 0.0;
           if (k < m && b < nd_c && t >= t0 && t < t1) {
-            c[k] = __builtin_nontemporal_load(codes + t * nd_c + b);
-            v[k] = (double)__builtin_nontemporal_load(E + (t - t0) * nd_c + b);
+            c[k] = codes[t * nd_c + b];
+            v[k] = (double)E[(t - t0) * nd_c + b];
           }
         }
+        // one LDS add a run: a wavefront's LDS operations complete in issue order, so the
+        // adds of run k land before those of run k + 1 and every sum keeps its order
 #pragma unroll
-        for (int k = 0; k < BATCH; ++k) {
-          if (c[k] != DROP) buf[c[k]] += v[k];
-          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-          __builtin_amdgcn_wave_barrier();
-        }
+        for (int k = 0; k < BATCH; ++k)
+          if (c[k] != DROP) atomicAdd(&buf[c[k]], v[k]);
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
       }
     }
     for (int i = lane; i < len; i += LPR) data[lo + i] = (OT)buf[i];
