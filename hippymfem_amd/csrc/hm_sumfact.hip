@@ -69,6 +69,42 @@ __host__ __device__ inline int sf_element_words(int npair, int nd_u, int nd_m) {
   return (w + 1) & ~1;
 }
 
+// The point state of point q from LDS: the element's vertex coordinates X, dof values U, M, P,
+// and the tables stored point-minor (entry (row, q) at row * 27 + q: lanes reading their own
+// points hit consecutive words) -- TX[(v * 3 + k)][q], TM[a][q], TU: N[a][q] then G[(a * 3 + k)][q].
+__device__ inline void sf_point_state(const hm_problem& P, int q, const double* X, const double* U,
+                                      const double* M, const double* Pv, bool fields, const double* TX,
+                                      const double* TM, const double* TU, PointState& S) {
+  double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll 2
+  for (int v = 0; v < 8; ++v)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double g = TX[(v * 3 + k) * 27 + q];
+      J[0][k] = fma(X[v * 3], g, J[0][k]);
+      J[1][k] = fma(X[v * 3 + 1], g, J[1][k]);
+      J[2][k] = fma(X[v * 3 + 2], g, J[2][k]);
+    }
+  const int nu = P.nd_u, nm = P.nd_m;
+  double uq = 0, mq = 0, ru[3] = {0, 0, 0}, rp[3] = {0, 0, 0};
+  if (fields) {
+    const double* G = TU + 27 * nu;
+#pragma unroll 3
+    for (int a = 0; a < nu; ++a) {
+      const double ua = U[a], pa = Pv[a];
+      uq = fma(TU[a * 27 + q], ua, uq);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const double g = G[(a * 3 + k) * 27 + q];
+        ru[k] = fma(g, ua, ru[k]);
+        rp[k] = fma(g, pa, rp[k]);
+      }
+    }
+  }
+  for (int a = 0; a < nm; ++a) mq = fma(TM[a * 27 + q], M[a], mq);
+  point_finish(P, q, J, uq, ru, rp, mq, S);
+}
+
 // LDS words of the tables every element reads: the trilinear map's gradients [27][8][3], the
 // parameter basis [27][nd_m] and, when the fields are read, the state basis and its gradients
 // [27][nd_u] + [27][nd_u][3].
@@ -77,7 +113,9 @@ __host__ __device__ inline int sf_table_words(int nd_u, int nd_m, bool fields) {
 }
 
 template <typename CT, int NT, int NC, int QP>
-__global__ void __launch_bounds__(256) k_sumfact(const hm_problem P, int s, int j, int64_t e0, int64_t e1,
+// Three wavefronts a SIMD: the register budget that gives (a few registers spill on the
+// FP64 Q2 forms; measured faster than two wavefronts without spills).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) k_sumfact(const hm_problem P, int s, int j, int64_t e0, int64_t e1,
                                                  SumfactPlan SP, const double* __restrict__ t1t,
                                                  const double* __restrict__ t1c, CT* __restrict__ out) {
   constexpr int LPE = NT * NC * NT;          // lanes an element
@@ -87,6 +125,7 @@ __global__ void __launch_bounds__(256) k_sumfact(const hm_problem P, int s, int 
   // the products of the 1-D tables, PT[dt][dc][a][b][q] = Xt_dt(a, q) Xc_dc(b, q), d the
   // derivative flags: every factor of stages 1, 2 and 3
   __shared__ double PT[4 * NT * NC * 3];
+  __shared__ int32_t LX[nd_t + nd_c];  // local dof of each tensor index, test then trial
   extern __shared__ double lds[];
   const int npair = SP.npair, nu = P.nd_u, nm = P.nd_m;
   const bool fields = SP.need_fields != 0;
@@ -94,22 +133,30 @@ __global__ void __launch_bounds__(256) k_sumfact(const hm_problem P, int s, int 
   double* TM = TX + 27 * 24;
   double* TU = TM + 27 * nm;
   {
+    const int32_t* lxt = s == HM_PARAMETER ? P.lex_m : P.lex_u;
+    const int32_t* lxc = j == HM_PARAMETER ? P.lex_m : P.lex_u;
+    for (int i = threadIdx.x; i < nd_t; i += blockDim.x) LX[i] = lxt[i];
+    for (int i = threadIdx.x; i < nd_c; i += blockDim.x) LX[nd_t + i] = lxc[i];
     for (int i = threadIdx.x; i < 4 * NT * NC * 3; i += blockDim.x) {
       const int q = i % 3, b = (i / 3) % NC, a = (i / (3 * NC)) % NT, dd = i / (3 * NC * NT);
       PT[i] = t1t[((dd >> 1) * NT + a) * 3 + q] * t1c[((dd & 1) * NC + b) * 3 + q];
     }
-    for (int i = threadIdx.x; i < 27 * 24; i += blockDim.x) TX[i] = P.tab_x[i];
-    for (int i = threadIdx.x; i < 27 * nm; i += blockDim.x) TM[i] = P.tab_m[i];
-    if (fields)
-      for (int i = threadIdx.x; i < 27 * nu * 4; i += blockDim.x) TU[i] = P.tab_u[i];
+    // the tables point-minor: entry (row, q) at row * 27 + q
+    for (int i = threadIdx.x; i < 27 * 24; i += blockDim.x) TX[(i % 24) * 27 + i / 24] = P.tab_x[i];
+    for (int i = threadIdx.x; i < 27 * nm; i += blockDim.x) TM[(i % nm) * 27 + i / nm] = P.tab_m[i];
+    if (fields) {
+      for (int i = threadIdx.x; i < 27 * nu; i += blockDim.x) TU[(i % nu) * 27 + i / nu] = P.tab_u[i];
+      for (int i = threadIdx.x; i < 27 * nu * 3; i += blockDim.x)
+        TU[27 * nu + (i % (nu * 3)) * 27 + i / (nu * 3)] = P.tab_u[27 * nu + i];
+    }
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int slot = lane / LPE, r = lane % LPE;
   const bool in_use = slot < EPW;
   const int a1 = r % NT, b1 = (r / NT) % NC, a2 = r / (NT * NC);
-  const int32_t* lt = s == HM_PARAMETER ? P.lex_m : P.lex_u;
-  const int32_t* lc = j == HM_PARAMETER ? P.lex_m : P.lex_u;
+  const int32_t* lt = LX;
+  const int32_t* lc = LX + nd_t;
   double* Wl = lds + sf_table_words(nu, nm, fields) +
                (size_t)(wave * EPW + (in_use ? slot : 0)) * sf_element_words(npair, nu, nm);
   double* Xl = Wl + npair * 27;
@@ -118,37 +165,69 @@ __global__ void __launch_bounds__(256) k_sumfact(const hm_problem P, int s, int 
   double* Ml = Pl + nu;
   const int64_t ne = e1 - e0;
   const int64_t stride = (int64_t)gridDim.x * SF_WPB * EPW;
-  // the next element's vertex coordinates and dof values, in flight in registers while the
-  // current element is computed: lane r holds vertex r and dofs r, r + LPE, ...
+  // The next element's data travels in two steps, so that no wait falls on the stores of an
+  // element matrix (a wait for a load also waits for the memory operations issued before it):
+  // its ids (vertex, dofs) are loaded early in an iteration, its vertex coordinates and dof
+  // values after the contraction and before the stores, and they are put in LDS at the top of
+  // the next iteration.  Lane r holds vertex r and dofs r, r + LPE, ...
+  int32_t ix = 0, iu[NPF], im[NPF];
   double fx[3], fu[NPF], fp[NPF], fm[NPF];
-  auto fetch = [&](int64_t el) {
+  auto fetch_ids = [&](int64_t el) {
     const bool ok = in_use && el < ne;
     const int64_t e = e0 + (ok ? el : 0);
+    if (ok && r < 8) ix = P.elem_verts[e * 8 + r];
+#pragma unroll
+    for (int k = 0; k < NPF; ++k) {
+      const int a = r + k * LPE;
+      if (ok && fields && a < nu) iu[k] = P.edofs_u[e * nu + a];
+      if (ok && a < nm) im[k] = P.edofs_m[e * nm + a];
+    }
+  };
+  auto fetch_values = [&](int64_t el) {
+    const bool ok = in_use && el < ne;
     if (ok && r < 8) {
-      const int64_t iv = P.elem_verts[e * 8 + r];
-      fx[0] = P.verts[iv * 3];
-      fx[1] = P.verts[iv * 3 + 1];
-      fx[2] = P.verts[iv * 3 + 2];
+      fx[0] = P.verts[(int64_t)ix * 3];
+      fx[1] = P.verts[(int64_t)ix * 3 + 1];
+      fx[2] = P.verts[(int64_t)ix * 3 + 2];
     }
 #pragma unroll
     for (int k = 0; k < NPF; ++k) {
       const int a = r + k * LPE;
       if (ok && fields && a < nu) {
-        const int64_t d = P.edofs_u[e * nu + a];
-        fu[k] = P.u[d];
-        fp[k] = P.p[d];
+        fu[k] = P.u[iu[k]];
+        fp[k] = P.p[iu[k]];
       }
-      if (ok && a < nm) fm[k] = P.m[P.edofs_m[e * nm + a]];
+      if (ok && a < nm) fm[k] = P.m[im[k]];
+    }
+  };
+  // the element matrix of the previous iteration (-1: none), written once the next element's
+  // values have arrived, so the stores never hold up a wait
+  constexpr bool LOWP = QP == HM_BF16 || QP == HM_FP16;
+  CT acc[NC][NT][NC];
+  int64_t prev_el = -1;
+  double prev_scale = 1.0;
+  auto store_prev = [&]() {
+    if (prev_el < 0) return;
+    CT* Eo = out + (size_t)prev_el * nd_t * nd_c;
+#pragma unroll
+    for (int a3 = 0; a3 < NT; ++a3) {
+      CT* Er = Eo + lt[a1 + NT * (a2 + NT * a3)] * nd_c;
+#pragma unroll
+      for (int b2 = 0; b2 < NC; ++b2)
+#pragma unroll
+        for (int b3 = 0; b3 < NC; ++b3)
+          Er[lc[b1 + NC * (b2 + NC * b3)]] = LOWP ? (CT)(acc[b2][a3][b3] * prev_scale) : acc[b2][a3][b3];
     }
   };
   int64_t base = ((int64_t)blockIdx.x * SF_WPB + wave) * EPW;
-  fetch(base + slot);
+  fetch_ids(base + slot);
+  fetch_values(base + slot);
   for (; base < ne; base += stride) {
     // the table reads stay inside the loop (hoisted, the products would take the registers)
     asm volatile("" ::: "memory");
     const int64_t el = base + slot;
     const bool act = in_use && el < ne;
-    // 1. the element's vertex coordinates and dof values into LDS; the next element's fetched
+    // 1. the element's vertex coordinates and dof values into LDS; the next element's ids
     if (act) {
       if (r < 8) {
         Xl[r * 3] = fx[0];
@@ -165,13 +244,14 @@ __global__ void __launch_bounds__(256) k_sumfact(const hm_problem P, int s, int 
         if (a < nm) Ml[a] = fm[k];
       }
     }
-    fetch(el + stride);
+    store_prev();
+    fetch_ids(el + stride);
     wave_sync();
     // 2. lane = quadrature point: the pointwise pairs into LDS, in the order of k = 4 r + c
     if (act)
       for (int q = r; q < 27; q += LPE) {
         PointState S;
-        point_state(P, q, Xl, Ul, Ml, Pl, S, fields, TX, TM, TU);
+        sf_point_state(P, q, Xl, Ul, Ml, Pl, fields, TX, TM, TU, S);
         double D[4][4];
         point_block(s, j, S, D);
         int p = 0;
@@ -210,9 +290,7 @@ __global__ void __launch_bounds__(256) k_sumfact(const hm_problem P, int s, int 
     const uint32_t live = (uint32_t)both & ((1u << npair) - 1u);
     // 4. lane = contraction task (a1, b1, a2).  The low-precision forms scale the element's
     // pairs by its largest pointwise weight (operands within [-1, 1]) and the result back.
-    constexpr bool LOWP = QP == HM_BF16 || QP == HM_FP16;
     const double wscale = (LOWP && top > 0.0) ? 1.0 / top : 1.0;
-    CT acc[NC][NT][NC];
 #pragma unroll
     for (int b2 = 0; b2 < NC; ++b2)
 #pragma unroll
@@ -231,10 +309,7 @@ __global__ void __launch_bounds__(256) k_sumfact(const hm_problem P, int s, int 
         const int d1 = ((SP.pr[p] == 1) << 1) | (SP.pc[p] == 1);
         const int d2 = ((SP.pr[p] == 2) << 1) | (SP.pc[p] == 2);
         // every LDS operand of the pair in flight at once (one wait, not one a use)
-        const double* w = Wl + p * 27;
-        double wv[27];
-#pragma unroll
-        for (int i = 0; i < 27; ++i) wv[i] = w[i];
+        const double* wv = Wl + p * 27;
         const double* f1 = PT + ((d1 * NT + a1) * NC + b1) * 3;
         const double g0d = f1[0], g1d = f1[1], g2d = f1[2];
         double hd[NC][3];
@@ -309,21 +384,15 @@ __global__ void __launch_bounds__(256) k_sumfact(const hm_problem P, int s, int 
         }
       }
     }
-    if (act) {
-      CT* Eo = out + (size_t)el * nd_t * nd_c;
-#pragma unroll
-      for (int a3 = 0; a3 < NT; ++a3) {
-        CT* Er = Eo + lt[a1 + NT * (a2 + NT * a3)] * nd_c;
-#pragma unroll
-        for (int b2 = 0; b2 < NC; ++b2)
-#pragma unroll
-          for (int b3 = 0; b3 < NC; ++b3)
-            Er[lc[b1 + NC * (b2 + NC * b3)]] = LOWP ? (CT)(acc[b2][a3][b3] * top) : acc[b2][a3][b3];
-      }
-    }
+    // the next element's values in flight; this element's matrix leaves at the top of the
+    // next iteration, after those values are waited for
+    fetch_values(el + stride);
+    prev_el = act ? el : -1;
+    prev_scale = top;
     // the LDS of this element is rewritten by the next
     wave_sync();
   }
+  store_prev();
 }
 
 template <typename CT, int NT, int NC, int QP>
