@@ -9,6 +9,9 @@
 
 namespace hm {
 
+// largest element dof count of the per-thread element arrays
+constexpr int HM_MAXND = 64;
+
 void set_error(const char* fmt, ...);
 
 }  // namespace hm

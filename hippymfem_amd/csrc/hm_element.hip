@@ -202,6 +202,58 @@ static int check_problem(const hm_problem* P) {
   return HM_OK;
 }
 
+// The consistent single-precision block (kernel.py:1503-1571): one thread per element.  K is
+// the element matrix in FP32; Rj = E c_j 1 and Ri = E^T c_i 1 are its exact (FP64) products
+// with the normalized constants of the two slots, c = 1 / sqrt(nd).  The result, in FP64, is K
+// corrected by rank-one terms so that its products with the constants are Rj and Ri:
+//   two-sided  K + Dj c_j 1^T + c_i 1 Di^T - c_i^2 c_j (1^T Dj) 1 1^T,
+//              Dj = Rj - c_j K 1,  Di = Ri - c_i K^T 1;
+//   symmetric  Ks + c (Dj 1^T + 1 Dj^T) - c^2 (c 1^T Dj) 1 1^T,  Ks = (K + K^T) / 2,
+//              Dj = Rj - c Ks 1, each value computed once for (a, b) and (b, a).
+__global__ void __launch_bounds__(256) k_consistent_block(const float* __restrict__ K,
+                                                          const double* __restrict__ Rj,
+                                                          const double* __restrict__ Ri, int64_t ne,
+                                                          int nd_i, int nd_j, int symmetric,
+                                                          double* __restrict__ out) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (int64_t)gridDim.x * blockDim.x) {
+    const float* Ke = K + (size_t)e * nd_i * nd_j;
+    double* Oe = out + (size_t)e * nd_i * nd_j;
+    const double cj = 1.0 / sqrt((double)nd_j), ci = 1.0 / sqrt((double)nd_i);
+    double Dj[HM_MAXND];
+    double sum_dj = 0.0;
+    if (symmetric) {
+      const int n = nd_i;
+      for (int a = 0; a < n; ++a) {
+        double row = 0.0;
+        for (int b = 0; b < n; ++b) row += 0.5 * ((double)Ke[a * n + b] + (double)Ke[b * n + a]);
+        Dj[a] = Rj[e * n + a] - cj * row;
+        sum_dj += Dj[a];
+      }
+      const double w = cj * cj * (cj * sum_dj);
+      for (int a = 0; a < n; ++a)
+        for (int b = a; b < n; ++b) {
+          const double v = 0.5 * ((double)Ke[a * n + b] + (double)Ke[b * n + a]) + cj * (Dj[a] + Dj[b]) - w;
+          Oe[a * n + b] = v;
+          Oe[b * n + a] = v;
+        }
+    } else {
+      for (int a = 0; a < nd_i; ++a) {
+        double row = 0.0;
+        for (int b = 0; b < nd_j; ++b) row += (double)Ke[a * nd_j + b];
+        Dj[a] = Rj[e * nd_i + a] - cj * row;
+        sum_dj += Dj[a];
+      }
+      const double w = ci * ci * cj * sum_dj;
+      for (int b = 0; b < nd_j; ++b) {
+        double col = 0.0;
+        for (int a = 0; a < nd_i; ++a) col += (double)Ke[a * nd_j + b];
+        const double Di = Ri[e * nd_j + b] - ci * col;
+        for (int a = 0; a < nd_i; ++a) Oe[a * nd_j + b] = (double)Ke[a * nd_j + b] + Dj[a] * cj + ci * Di - w;
+      }
+    }
+  }
+}
+
 static int grid_for(int64_t n) {
   const int64_t g = n < 1 ? 1 : n;
   return (int)(g < 65536 ? g : 65536);
@@ -248,6 +300,20 @@ extern "C" int hm_element_matrices(const hm_problem* P, int i, int j, int64_t e0
   hipLaunchKernelGGL(k_element_matrices<double>, dim3(grid_for(e1 - e0)), dim3(256), bytes, s, *P, i,
                      j, e0, e1, parts, (double*)out);
   return hip_check("hm_element_matrices");
+}
+
+extern "C" int hm_consistent_block(const float* K, const double* Rj, const double* Ri, int64_t ne, int nd_i,
+                                   int nd_j, int symmetric, double* out, void* stream) {
+  HM_REQUIRE(ne >= 0 && nd_i > 0 && nd_j > 0 && nd_i <= HM_MAXND && nd_j <= HM_MAXND,
+             "bad element shape (%d, %d)", nd_i, nd_j);
+  HM_REQUIRE(!symmetric || nd_i == nd_j, "a symmetric block is square");
+  HM_REQUIRE(symmetric || Ri != nullptr, "a two-sided correction needs the transposed product");
+  HM_REQUIRE(K && Rj && out, "null array");
+  if (ne == 0) return HM_OK;
+  const int64_t nb = (ne + 255) / 256;
+  hipLaunchKernelGGL(k_consistent_block, dim3((unsigned)(nb < 65536 ? nb : 65536)), dim3(256), 0,
+                     (hipStream_t)stream, K, Rj, Ri, ne, nd_i, nd_j, symmetric, out);
+  return hip_check("hm_consistent_block");
 }
 
 extern "C" int hm_element_vectors(const hm_problem* P, int i, int64_t e0, int64_t e1, double* out,

@@ -86,7 +86,13 @@ class QuadratureKernel:
                          precision="fp64", elements=None, out=None, stream=None):
         """Block ``(i, j)`` element matrices ``[(ne, nd_i, nd_j)]`` (``element_matrices``,
         ``kernel.py:1964-2006``).  ``precision``: ``"fp64"`` (default), or ``"fp32"``,
-        ``"bf16"``, ``"fp16"`` for the precision form (fp32 results)."""
+        ``"bf16"``, ``"fp16"`` for the precision form (fp32 results).  ``consistent``
+        (``"symmetric"`` or ``"two-sided"``) with a precision form: the consistent block of
+        ``_block_fn`` (``kernel.py:1573-1580``), fp64 matrices exact on the constants."""
+        if consistent and precision != "fp64":
+            return [self.consistent_block(i, j, local_arrays, consistent == "symmetric",
+                                          precision=precision, elements=elements, out=out,
+                                          stream=stream)]
         P, keep = self.problem(local_arrays)
         a, b = (0, self.group.ne) if elements is None else elements
         code = _lib.PRECISIONS[precision]
@@ -98,6 +104,34 @@ class QuadratureKernel:
             _lib.ctypes.byref(P), int(i), int(j), int(a), int(b), code, _lib.ptr(out),
             _lib.stream_handle(stream)), "element matrices (%d, %d)" % (i, j))
         return [out]
+
+    def consistent_block(self, i, j, local_arrays, symmetric=False, precision="fp32",
+                         elements=None, out=None, stream=None):
+        """Block ``(i, j)`` in single precision, exact on the constants
+        (``consistent_block``, ``kernel.py:1503-1571``): the precision-form element matrices
+        corrected by rank-one terms so that their products with the constants of slot ``j``
+        (and, transposed, of slot ``i``) are the FP64 ones, which come from the matrix-free
+        element products.  ``symmetric``: symmetrize first; the result is symmetric to the
+        last bit.  FP64 ``(ne, nd_i, nd_j)``."""
+        a, b = (0, self.group.ne) if elements is None else elements
+        K = self.element_matrices(i, j, local_arrays, precision=precision, elements=(a, b),
+                                  stream=stream)[0]
+        dev = K.device
+        ones_j = torch.full((self.spaces[j].ndof,), 1.0 / np.sqrt(self.nd[j]), dtype=torch.float64,
+                            device=dev)
+        Rj = self.element_hvp(i, j, local_arrays, ones_j, elements=(a, b), stream=stream)[0]
+        Ri = None
+        if not symmetric:
+            ones_i = torch.full((self.spaces[i].ndof,), 1.0 / np.sqrt(self.nd[i]),
+                                dtype=torch.float64, device=dev)
+            Ri = self.element_hvp(j, i, local_arrays, ones_i, elements=(a, b), stream=stream)[0]
+        if out is None:
+            out = torch.empty((b - a, self.nd[i], self.nd[j]), dtype=torch.float64, device=dev)
+        _lib.check(_lib.lib().hm_consistent_block(
+            _lib.ptr(K), _lib.ptr(Rj), _lib.ptr(Ri), b - a, self.nd[i], self.nd[j],
+            1 if symmetric else 0, _lib.ptr(out), _lib.stream_handle(stream)), "consistent block")
+        self._keep = (self._keep, K, Rj, Ri)
+        return out
 
     def element_matrices_many(self, pairs, local_arrays, precision="fp64"):
         """Several blocks from one call (``element_matrices_many``, ``kernel.py:2048-2068``)."""

@@ -91,6 +91,16 @@ typedef struct hm_problem {
 int hm_element_matrices(const hm_problem* P, int i, int j, int64_t e0, int64_t e1,
                         int precision, void* out, void* stream);
 
+/* The consistent single-precision block: K (ne, nd_i, nd_j) fp32 element matrices,
+ * Rj (ne, nd_i) and Ri (ne, nd_j) their exact fp64 products with the normalized constants of
+ * slot j (1/sqrt(nd_j) at every dof) and, transposed, of slot i.  out (ne, nd_i, nd_j) fp64 is
+ * K plus the rank-one corrections that make its products with those constants Rj and Ri;
+ * symmetric = 1 symmetrizes K first, takes Rj for both sides (Ri may be NULL) and gives a
+ * result symmetric to the last bit.  Replaces GroupKernel.consistent_block
+ * (hippymfem/fem/kernel.py:1503-1571) for scalar spaces (one constant mode a slot). */
+int hm_consistent_block(const float* K, const double* Rj, const double* Ri, int64_t ne, int nd_i,
+                        int nd_j, int symmetric, double* out, void* stream);
+
 /* Element vectors dR_e/d(slot i) for [e0, e1): out (e1-e0, nd_i) fp64.
  * Replaces GroupKernel.grad_block (hippymfem/fem/kernel.py:1343-1355). */
 int hm_element_vectors(const hm_problem* P, int i, int64_t e0, int64_t e1, double* out,

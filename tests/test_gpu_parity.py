@@ -279,3 +279,34 @@ def test_lowp_needs_tensor_structure(gpu):
     W = Workload(2, quadrature_degree=6)            # 4 points a direction: no sum factorization
     with pytest.raises(NotImplementedError):
         W.kernel.element_matrices(ADJOINT, STATE, W.locals(), precision="bf16")
+
+
+@pytest.mark.parametrize("nonlinear", [False, True])
+def test_consistent_block(gpu, nonlinear):
+    """The consistent single-precision block (FP32 contraction, rank-one FP64 correction):
+    its products with the constants are the FP64 ones (rows, and columns when two-sided);
+    the symmetric form is symmetric to the last bit; the entries stay within the FP32 form's
+    error of the oracle; the plain FP32 matrix misses the constants by far more."""
+    W, P = pair(4, perturb=0.2, nonlinear=nonlinear)
+    blocks = [((ADJOINT, STATE), not nonlinear), ((ADJOINT, PARAMETER), False),
+              ((STATE, PARAMETER), False)] + ([((STATE, STATE), True)] if nonlinear else [])
+    for (i, j), sym in blocks:
+        kind = "symmetric" if sym else "two-sided"
+        Ec = W.kernel.element_matrices(i, j, W.locals(), consistent=kind, precision="fp32")[0]
+        assert Ec.dtype == torch.float64
+        Ec = Ec.cpu().numpy()
+        Eo = P.block(i, j)
+        K = W.kernel.element_matrices(i, j, W.locals(), precision="fp32")[0].cpu().numpy()
+        scale = np.abs(Eo).max() * Eo.shape[2]
+        err_rows = np.abs(Ec.sum(axis=2) - Eo.sum(axis=2)).max() / scale
+        err_rows_fp32 = np.abs(K.astype(np.float64).sum(axis=2) - Eo.sum(axis=2)).max() / scale
+        print("%s (%d, %d): rows %.1e (fp32 %.1e)" % (kind, i, j, err_rows, err_rows_fp32),
+              flush=True)
+        assert err_rows <= 1e-14, (i, j, err_rows)
+        assert err_rows_fp32 > 10 * err_rows
+        if sym:
+            assert np.array_equal(Ec, Ec.transpose(0, 2, 1)), (i, j)
+        else:
+            err_cols = np.abs(Ec.sum(axis=1) - Eo.sum(axis=1)).max() / (np.abs(Eo).max() * Eo.shape[1])
+            assert err_cols <= 1e-14, (i, j, err_cols)
+        assert rel(Ec, Eo) <= 1e-6, (i, j)
