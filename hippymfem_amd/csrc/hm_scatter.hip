@@ -29,7 +29,8 @@ struct Drop<uint16_t> {
   static constexpr unsigned v = 0xFFFFu;
 };
 
-// LPR lanes reduce one row; a block of 256 threads holds 256/LPR row buffers.  The runs
+// LPR lanes reduce one row (LPR a power of two up to 64, the element rows' entries taken LPR at
+// a time); a block of 256 threads holds 256/LPR row buffers.  The runs
 // of a row are taken eight at a time: their ids are read by one load and spread with a
 // lane broadcast, all eight runs' entries and codes are loaded before any is added (eight
 // loads in flight per lane), then added run after run in ascending order.
@@ -127,24 +128,31 @@ __global__ void __launch_bounds__(256) k_csr_matvec(int64_t nrow, const int32_t*
   }
 }
 
+template <typename CodeT, typename ET, typename OT, int LPR>
+static void launch_lpr(const hm_scatter_plan& S, const void* E, int64_t t0, int64_t t1, void* data,
+                       int accumulate, hipStream_t st) {
+  constexpr int groups = 256 / LPR;
+  const int64_t nb = (S.nrow + groups - 1) / groups;
+  const int grid = (int)(nb < 131072 ? (nb < 1 ? 1 : nb) : 131072);
+  hipLaunchKernelGGL((k_scatter_rows<CodeT, ET, OT, LPR>), dim3(grid), dim3(256),
+                     (size_t)groups * S.max_row * sizeof(double), st, S, (const ET*)E, t0, t1, (OT*)data,
+                     accumulate);
+}
+
+// Lanes a row (measured on MI355X at 64^3): as many as an element row has entries for rows of
+// up to 8 (more rows in flight, the reduction is latency-bound); 32 for FP64 element rows of
+// up to 32 entries, 16 for FP32 ones (half the bytes a load, two loads a run).
 template <typename CodeT, typename ET, typename OT>
 static void launch_rows(const hm_scatter_plan& S, const void* E, int64_t t0, int64_t t1, void* data,
                         int accumulate, hipStream_t st) {
-  if (S.nd_c <= 32) {
-    constexpr int groups = 256 / 32;
-    const int64_t nb = (S.nrow + groups - 1) / groups;
-    const int grid = (int)(nb < 131072 ? (nb < 1 ? 1 : nb) : 131072);
-    hipLaunchKernelGGL((k_scatter_rows<CodeT, ET, OT, 32>), dim3(grid), dim3(256),
-                       (size_t)groups * S.max_row * sizeof(double), st, S, (const ET*)E, t0, t1,
-                       (OT*)data, accumulate);
-  } else {
-    constexpr int groups = 256 / 64;
-    const int64_t nb = (S.nrow + groups - 1) / groups;
-    const int grid = (int)(nb < 131072 ? (nb < 1 ? 1 : nb) : 131072);
-    hipLaunchKernelGGL((k_scatter_rows<CodeT, ET, OT, 64>), dim3(grid), dim3(256),
-                       (size_t)groups * S.max_row * sizeof(double), st, S, (const ET*)E, t0, t1,
-                       (OT*)data, accumulate);
-  }
+  if (S.nd_c <= 8)
+    launch_lpr<CodeT, ET, OT, 8>(S, E, t0, t1, data, accumulate, st);
+  else if (S.nd_c <= 16 || (S.nd_c <= 32 && sizeof(ET) == 4))
+    launch_lpr<CodeT, ET, OT, 16>(S, E, t0, t1, data, accumulate, st);
+  else if (S.nd_c <= 32)
+    launch_lpr<CodeT, ET, OT, 32>(S, E, t0, t1, data, accumulate, st);
+  else
+    launch_lpr<CodeT, ET, OT, 64>(S, E, t0, t1, data, accumulate, st);
 }
 
 template <typename CodeT>
