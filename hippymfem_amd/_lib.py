@@ -43,7 +43,11 @@ class HmScatterPlan(ctypes.Structure):
     """``hm_scatter_plan`` of ``include/hippymfem_amd.h``."""
     _fields_ = [("nrow", I64), ("nnz", I64), ("ne", I64), ("nd_t", I32), ("nd_c", I32),
                 ("code_bytes", I32), ("max_row", I32), ("indptr", P), ("rptr", P),
-                ("runs", P), ("codes", P), ("rowmode", P), ("diag", DBL)]
+                ("runs", P), ("codes", P), ("rowmode", P), ("diag", DBL), ("rowhead", P)]
+
+
+#: int32 words of a row header (``HM_ROWHEAD``)
+ROWHEAD = 12
 
 
 _LIB = None

@@ -47,10 +47,27 @@ __global__ void __launch_bounds__(256) k_scatter_rows(hm_scatter_plan S, const E
   const CodeT* codes = (const CodeT*)S.codes;
   const int nd_c = S.nd_c;
   constexpr unsigned DROP = Drop<CodeT>::v;
+  // with row headers (LPR >= 16), a row's tables and first eight run ids arrive in one load
+  const bool rh = LPR >= 16 && S.rowhead != nullptr;
   for (int64_t r = (int64_t)blockIdx.x * groups + grp; r < S.nrow; r += (int64_t)gridDim.x * groups) {
-    const int32_t lo = S.indptr[r];
-    const int32_t len = S.indptr[r + 1] - lo;
-    const int mode = S.rowmode ? S.rowmode[r] : 0;
+    int32_t lo, len, p0, p1, head = 0;
+    int mode;
+    if (rh) {
+      head = lane < HM_ROWHEAD ? S.rowhead[r * HM_ROWHEAD + lane] : 0;
+      const int32_t packed = __shfl(head, 1, LPR);
+      lo = __shfl(head, 0, LPR);
+      len = packed & 0xFFFF;
+      mode = (packed >> 16) & 0xFF;
+      p0 = __shfl(head, 2, LPR);
+      p1 = p0 + ((packed >> 24) & 0xFF);
+      head = __shfl(head, 3 + (lane & 7), LPR);  // lane k < 8: the row's run k
+    } else {
+      lo = S.indptr[r];
+      len = S.indptr[r + 1] - lo;
+      mode = S.rowmode ? S.rowmode[r] : 0;
+      p0 = S.rptr[r];
+      p1 = S.rptr[r + 1];
+    }
     if (mode != 0) {
       for (int i = lane; i < len; i += LPR) data[lo + i] = (OT)((mode == 2 && i == 0) ? S.diag : 0.0);
       continue;
@@ -58,10 +75,9 @@ __global__ void __launch_bounds__(256) k_scatter_rows(hm_scatter_plan S, const E
     for (int i = lane; i < len; i += LPR) buf[i] = accumulate ? (double)data[lo + i] : 0.0;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    const int32_t p1 = S.rptr[r + 1];
-    for (int32_t p = S.rptr[r]; p < p1; p += BATCH) {
+    for (int32_t p = p0; p < p1; p += BATCH) {
       const int m = p1 - p < BATCH ? p1 - p : BATCH;
-      const int32_t mine = lane < m ? S.runs[p + lane] : -1;
+      const int32_t mine = (rh && p == p0) ? head : (lane < m ? S.runs[p + lane] : -1);
       for (int b0 = 0; b0 < nd_c; b0 += LPR) {
         const int b = b0 + lane;
         double v[BATCH];

@@ -181,15 +181,39 @@ class ScatterPattern:
                                        % (int(np.count_nonzero(self.indices[first] != rows)),
                                           rows.size))
             rowmode = torch.from_numpy((rm.astype(np.int8) * (2 if diag is not None else 1))).to(dev)
+        head = self._row_headers(base, rowmode) if self.nd_c > 8 else None
         plan = _lib.HmScatterPlan(
             nrow=self.nrow, nnz=self.nnz, ne=self.ne, nd_t=self.nd_t, nd_c=self.nd_c,
             code_bytes=2 if wide else 1, max_row=self.max_row,
             indptr=_lib.ptr(base["indptr"]), rptr=_lib.ptr(base["rptr"]),
             runs=_lib.ptr(base["runs"]), codes=_lib.ptr(codes[1]),
-            rowmode=_lib.ptr(rowmode), diag=0.0 if diag is None else float(diag))
-        got = dict(plan=plan, keep=(base, codes, rowmode), row_mask=row_mask, col_mask=col_mask)
+            rowmode=_lib.ptr(rowmode), diag=0.0 if diag is None else float(diag),
+            rowhead=_lib.ptr(head))
+        got = dict(plan=plan, keep=(base, codes, rowmode, head), row_mask=row_mask,
+                   col_mask=col_mask)
         self._plans[key] = got
         return got
+
+    def _row_headers(self, base, rowmode):
+        """Every row's scatter tables packed in one 48-byte record (``hm_scatter_plan``'s
+        ``rowhead``): offset; length, mode and run count in one word; first run position; the
+        first eight run ids."""
+        ip = base["indptr"].long()
+        rp = base["rptr"].long()
+        runs = base["runs"]
+        nrun = rp[1:] - rp[:-1]
+        if self.max_row >= 1 << 16 or int(nrun.max()) >= 1 << 7:
+            return None
+        head = torch.full((self.nrow, _lib.ROWHEAD), -1, dtype=torch.int32, device=ip.device)
+        mode = rowmode.long() if rowmode is not None else torch.zeros_like(nrun)
+        head[:, 0] = ip[:-1].int()
+        head[:, 1] = ((ip[1:] - ip[:-1]) | (mode << 16) | (nrun << 24)).int()
+        head[:, 2] = rp[:-1].int()
+        for k in range(8):
+            has = nrun > k
+            idx = torch.where(has, rp[:-1] + k, torch.zeros_like(rp[:-1]))
+            head[:, 3 + k] = torch.where(has, runs[idx], torch.full_like(runs[idx], -1))
+        return head
 
     # ------------------------------------------------------------------ values
     def data(self, element_matrices, zero_slots=None, host=False, dtype=None, plan=None,

@@ -151,7 +151,14 @@ typedef struct hm_scatter_plan {
   const void* codes;
   const int8_t* rowmode;
   double diag;
+  /* Optional (NULL: not used), for element rows of more than 8 entries: the same row tables
+   * packed HM_ROWHEAD int32 a row, so that a row's arrive in one load: indptr[r];
+   * (indptr[r+1] - indptr[r]) | rowmode[r] << 16 | (rptr[r+1] - rptr[r]) << 24; rptr[r];
+   * runs[rptr[r] + k] for k < 8 (-1 past the row's runs); one unused.  A row of more than 8
+   * runs reads the others from runs. */
+  const int32_t* rowhead;
 } hm_scatter_plan;
+enum { HM_ROWHEAD = 12 };
 
 /* data[nnz] (+)= the entries of the element matrices of elements [e0, e1) (E holds
  * exactly those, (e1-e0, nd_t, nd_c)).  e_dtype / out_dtype: HM_FP64 or HM_FP32.
