@@ -33,13 +33,15 @@ struct SumfactPlan {
   int kmask;                           // bit 4 r + c set for every pair, pairs in the order of k
   int pr[SF_MAXPAIR], pc[SF_MAXPAIR];  // (test part r, trial part c) of each pair
   int tmask[4];                        // pairs of each third-direction type (r == 3, c == 3)
+  int offdiag;                         // the pairs of two different reference derivatives
   int need_fields;                     // the pointwise blocks read u or p
 };
 
-// A pair whose pointwise weight stays below this fraction of the element's largest is left out
-// of that element's contraction: its share of every entry is below the rounding of the sum.
-// On axis-aligned elements that drops the six off-diagonal gradient pairs of Jinv Jinv^T,
-// which are round-off there.
+// Pairs left out of an element's contraction, their share of every entry being below the
+// rounding of the sum: on an axis-aligned box element (J diagonal) the pairs of two different
+// reference derivatives, whose weights (Jinv Jinv^T off the diagonal) are round-off there; in
+// the low-precision forms, any pair whose pointwise weight stays below SF_DROP of the element's
+// largest.
 #define SF_DROP 1e-16
 
 constexpr int SF_WPB = 4;  // wavefronts a block
@@ -171,7 +173,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
   // values after the contraction and before the stores, and they are put in LDS at the top of
   // the next iteration.  Lane r holds vertex r and dofs r, r + LPE, ...
   int32_t ix = 0, iu[NPF], im[NPF];
-  double fx[3], fu[NPF], fp[NPF], fm[NPF];
+  double fx[3] = {0.0, 0.0, 0.0}, fu[NPF], fp[NPF], fm[NPF];
   auto fetch_ids = [&](int64_t el) {
     const bool ok = in_use && el < ne;
     const int64_t e = e0 + (ok ? el : 0);
@@ -244,6 +246,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
         if (a < nm) Ml[a] = fm[k];
       }
     }
+    // Is the element an axis-aligned box (every vertex coordinate that of the vertex with
+    // the same index along that axis)?  Then J is diagonal and the pairs of two different
+    // derivatives are round-off: they are left out of its contraction.
+    const uint64_t lpe_mask = (1ull << LPE) - 1ull;
+    const int sh = in_use ? slot * LPE : 0;
+    bool aligned;
+    {
+      const double rx = __shfl(fx[0], sh + (r & 1), 64), ry = __shfl(fx[1], sh + (r & 2), 64),
+                   rz = __shfl(fx[2], sh + (r & 4), 64);
+      const uint64_t bad = __ballot(act && r < 8 && !(fx[0] == rx && fx[1] == ry && fx[2] == rz));
+      aligned = ((bad >> sh) & lpe_mask) == 0;
+    }
     store_prev();
     fetch_ids(el + stride);
     wave_sync();
@@ -264,30 +278,27 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
           }
       }
     wave_sync();
-    // 3. the pairs this element needs: lane r measures pairs r and r + LPE against the
-    // element's largest pointwise weight
-    double m0 = 0.0, m1 = 0.0;
-    if (act && r < npair) {
-      double v[27];
-#pragma unroll
-      for (int q = 0; q < 27; ++q) v[q] = Wl[r * 27 + q];
-      __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-      for (int q = 0; q < 27; ++q) m0 = fmax(m0, fabs(v[q]));
+    // 3. the pairs this element needs.  The low-precision forms also measure every pair
+    // against the element's largest pointwise weight, their scale, and leave out the pairs
+    // below SF_DROP of it.
+    uint32_t live = ((1u << npair) - 1u) & (aligned ? ~(uint32_t)SP.offdiag : ~0u);
+    double top = 1.0;
+    if constexpr (LOWP) {
+      double m0 = 0.0, m1 = 0.0;
+      if (act && r < npair)
+        for (int q = 0; q < 27; ++q) m0 = fmax(m0, fabs(Wl[r * 27 + q]));
+      if (act && LPE + r < npair)
+        for (int q = 0; q < 27; ++q) m1 = fmax(m1, fabs(Wl[(LPE + r) * 27 + q]));
+      top = 0.0;
+      for (int p = 0; p < npair; ++p) {
+        const double v0 = __shfl(m0, slot * LPE + p % LPE, 64), v1 = __shfl(m1, slot * LPE + p % LPE, 64);
+        top = fmax(top, p < LPE ? v0 : v1);
+      }
+      const uint64_t bal0 = __ballot(act && r < npair && m0 > SF_DROP * top);
+      const uint64_t bal1 = __ballot(act && LPE + r < npair && m1 > SF_DROP * top);
+      const uint64_t both = ((bal0 >> sh) & lpe_mask) | (((bal1 >> sh) & lpe_mask) << LPE);
+      live = (uint32_t)both & ((1u << npair) - 1u);
     }
-    if (act && LPE + r < npair)
-      for (int q = 0; q < 27; ++q) m1 = fmax(m1, fabs(Wl[(LPE + r) * 27 + q]));
-    double top = 0.0;
-    for (int p = 0; p < npair; ++p) {
-      const double v0 = __shfl(m0, slot * LPE + p % LPE, 64), v1 = __shfl(m1, slot * LPE + p % LPE, 64);
-      top = fmax(top, p < LPE ? v0 : v1);
-    }
-    const uint64_t bal0 = __ballot(act && r < npair && m0 > SF_DROP * top);
-    const uint64_t bal1 = __ballot(act && LPE + r < npair && m1 > SF_DROP * top);
-    const uint64_t lpe_mask = (1ull << LPE) - 1ull;
-    const int sh = in_use ? slot * LPE : 0;
-    const uint64_t both = ((bal0 >> sh) & lpe_mask) | (((bal1 >> sh) & lpe_mask) << LPE);
-    const uint32_t live = (uint32_t)both & ((1u << npair) - 1u);
     // 4. lane = contraction task (a1, b1, a2).  The low-precision forms scale the element's
     // pairs by its largest pointwise weight (operands within [-1, 1]) and the result back.
     const double wscale = (LOWP && top > 0.0) ? 1.0 / top : 1.0;
@@ -487,7 +498,10 @@ int hm_element_matrices_sumfact(const hm_problem* P, int i, int j, int64_t e0, i
     const size_t n = (size_t)(e1 - e0) * (nt * nt * nt) * (nc * nc * nc);
     return hm_fill_zero(out, n, precision == HM_FP64 ? 8 : 4, (hipStream_t)stream);
   }
-  for (int p = 0; p < np; ++p) SP.tmask[((SP.pr[p] == 3) << 1) | (SP.pc[p] == 3)] |= 1 << p;
+  for (int p = 0; p < np; ++p) {
+    SP.tmask[((SP.pr[p] == 3) << 1) | (SP.pc[p] == 3)] |= 1 << p;
+    if (SP.pr[p] >= 1 && SP.pc[p] >= 1 && SP.pr[p] != SP.pc[p]) SP.offdiag |= 1 << p;
+  }
   SP.need_fields = !(i * 3 + j == 6 && !nl);  // the linear Jacobian reads m and the geometry only
   hipStream_t st = (hipStream_t)stream;
   switch (precision) {
