@@ -10,11 +10,11 @@
 // diffusion density it takes 17.5k multiply-adds an element, or 7k where the element is
 // axis-aligned, against 59k for the dense contraction.
 //
-// A wavefront holds 64 / (NT NC NT) elements.  The lanes of an element first stage its vertex
+// A wavefront holds 64 / (NT NC max(NT, NC)) elements.  The lanes of an element first stage its vertex
 // coordinates and dof values in LDS, then act as its quadrature points and write the block's
-// pointwise pairs W[p][q] (D^ of pair p at point q, FP64) to LDS, then act as its (a1, b1, a2)
-// contraction tasks: each runs stages 1 and 2 for its (a1, b1, a2) in registers and stage 3
-// into its NC NT NC entries (b2, a3, b3) of the element matrix, which it writes to memory.
+// pointwise pairs W[p][q] (D^ of pair p at point q, FP64) to LDS, then act as its contraction
+// tasks (a1, b1, x2), x2 = b2 or a2: each runs stages 1 and 2 in registers and stage 3 into its
+// entries (a2 or b2, a3, b3) of the element matrix, which it writes to memory.
 // The pointwise pairs never leave the compute unit.
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
@@ -120,7 +120,12 @@ template <typename CT, int NT, int NC, int QP>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) k_sumfact(const hm_problem P, int s, int j, int64_t e0, int64_t e1,
                                                  SumfactPlan SP, const double* __restrict__ t1t,
                                                  const double* __restrict__ t1c, CT* __restrict__ out) {
-  constexpr int LPE = NT * NC * NT;          // lanes an element
+  // Lane task (a1, b1, x2): x2 = b2 with the a2 loop in registers where NC >= NT (for a fixed
+  // entry index the lanes of one a1 then write NC^2 adjacent columns), x2 = a2 with the b2 loop
+  // otherwise (fewer registers for the wide-by-narrow blocks).
+  constexpr bool BMAP = NC >= NT;
+  constexpr int N2 = BMAP ? NT : NC;          // the stage-2 index held in registers
+  constexpr int LPE = NT * NC * (BMAP ? NC : NT);  // lanes an element
   constexpr int EPW = 64 / LPE;              // elements a wavefront
   constexpr int NPF = (27 + LPE - 1) / LPE;  // dofs a lane fetches
   constexpr int nd_t = NT * NT * NT, nd_c = NC * NC * NC;
@@ -156,7 +161,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int slot = lane / LPE, r = lane % LPE;
   const bool in_use = slot < EPW;
-  const int a1 = r % NT, b1 = (r / NT) % NC, a2 = r / (NT * NC);
+  const int a1 = r % NT, b1 = (r / NT) % NC, x2 = r / (NT * NC);
+  // the (a2, b2) of register index i
+  auto a2_of = [&](int i) { return BMAP ? i : x2; };
+  auto b2_of = [&](int i) { return BMAP ? x2 : i; };
   const int32_t* lt = LX;
   const int32_t* lc = LX + nd_t;
   double* Wl = lds + sf_table_words(nu, nm, fields) +
@@ -205,21 +213,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
   // the element matrix of the previous iteration (-1: none), written once the next element's
   // values have arrived, so the stores never hold up a wait
   constexpr bool LOWP = QP == HM_BF16 || QP == HM_FP16;
-  CT acc[NC][NT][NC];
+  CT acc[N2][NT][NC];
   int64_t prev_el = -1;
   double prev_scale = 1.0;
   auto store_prev = [&]() {
     if (prev_el < 0) return;
     CT* Eo = out + (size_t)prev_el * nd_t * nd_c;
 #pragma unroll
-    for (int a3 = 0; a3 < NT; ++a3) {
-      CT* Er = Eo + lt[a1 + NT * (a2 + NT * a3)] * nd_c;
+    for (int a3 = 0; a3 < NT; ++a3)
 #pragma unroll
-      for (int b2 = 0; b2 < NC; ++b2)
+      for (int i = 0; i < N2; ++i) {
+        CT* Er = Eo + lt[a1 + NT * (a2_of(i) + NT * a3)] * nd_c;
 #pragma unroll
         for (int b3 = 0; b3 < NC; ++b3)
-          Er[lc[b1 + NC * (b2 + NC * b3)]] = LOWP ? (CT)(acc[b2][a3][b3] * prev_scale) : acc[b2][a3][b3];
-    }
+          Er[lc[b1 + NC * (b2_of(i) + NC * b3)]] = LOWP ? (CT)(acc[i][a3][b3] * prev_scale) : acc[i][a3][b3];
+      }
   };
   int64_t base = ((int64_t)blockIdx.x * SF_WPB + wave) * EPW;
   fetch_ids(base + slot);
@@ -299,22 +307,22 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
       const uint64_t both = ((bal0 >> sh) & lpe_mask) | (((bal1 >> sh) & lpe_mask) << LPE);
       live = (uint32_t)both & ((1u << npair) - 1u);
     }
-    // 4. lane = contraction task (a1, b1, a2).  The low-precision forms scale the element's
+    // 4. lane = contraction task (a1, b1, x2).  The low-precision forms scale the element's
     // pairs by its largest pointwise weight (operands within [-1, 1]) and the result back.
     const double wscale = (LOWP && top > 0.0) ? 1.0 / top : 1.0;
 #pragma unroll
-    for (int b2 = 0; b2 < NC; ++b2)
+    for (int i = 0; i < N2; ++i)
 #pragma unroll
       for (int a3 = 0; a3 < NT; ++a3)
 #pragma unroll
-        for (int b3 = 0; b3 < NC; ++b3) acc[b2][a3][b3] = 0;
+        for (int b3 = 0; b3 < NC; ++b3) acc[i][a3][b3] = 0;
 #pragma unroll 1
     for (int t = 0; t < 4; ++t) {
       const uint32_t tm = live & (uint32_t)SP.tmask[t];
       if (!act || tm == 0) continue;
-      CT s2[NC][3];
+      CT s2[N2][3];
 #pragma unroll
-      for (int b2 = 0; b2 < NC; ++b2) s2[b2][0] = s2[b2][1] = s2[b2][2] = 0;
+      for (int i = 0; i < N2; ++i) s2[i][0] = s2[i][1] = s2[i][2] = 0;
       for (int p = 0; p < npair; ++p) {
         if (!((tm >> p) & 1)) continue;
         const int d1 = ((SP.pr[p] == 1) << 1) | (SP.pc[p] == 1);
@@ -323,19 +331,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
         const double* wv = Wl + p * 27;
         const double* f1 = PT + ((d1 * NT + a1) * NC + b1) * 3;
         const double g0d = f1[0], g1d = f1[1], g2d = f1[2];
-        double hd[NC][3];
+        double hd[N2][3];
 #pragma unroll
-        for (int b2 = 0; b2 < NC; ++b2) {
-          const double* f2 = PT + ((d2 * NT + a2) * NC + b2) * 3;
-          hd[b2][0] = f2[0];
-          hd[b2][1] = f2[1];
-          hd[b2][2] = f2[2];
+        for (int i = 0; i < N2; ++i) {
+          const double* f2 = PT + ((d2 * NT + a2_of(i)) * NC + b2_of(i)) * 3;
+          hd[i][0] = f2[0];
+          hd[i][1] = f2[1];
+          hd[i][2] = f2[2];
         }
         __builtin_amdgcn_sched_barrier(0);
         const CT g0 = quant<QP>((CT)g0d), g1 = quant<QP>((CT)g1d), g2 = quant<QP>((CT)g2d);
         // one q3 plane at a time:
         //   stage 1: s1[q2] = sum_q1 X(a1, q1) Y(b1, q1) W(q1, q2, q3)
-        //   stage 2: s2[b2][q3] += sum_q2 X(a2, q2) Y(b2, q2) s1[q2]
+        //   stage 2: s2[i][q3] += sum_q2 X(a2, q2) Y(b2, q2) s1[q2], (a2, b2) of i
 #pragma unroll
         for (int q3 = 0; q3 < 3; ++q3) {
           CT s1[3];
@@ -351,22 +359,22 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
             }
           }
 #pragma unroll
-          for (int b2 = 0; b2 < NC; ++b2)
-            s2[b2][q3] = fma(quant<QP>((CT)hd[b2][2]), s1[2],
-                             fma(quant<QP>((CT)hd[b2][1]), s1[1], fma(quant<QP>((CT)hd[b2][0]), s1[0], s2[b2][q3])));
+          for (int i = 0; i < N2; ++i)
+            s2[i][q3] = fma(quant<QP>((CT)hd[i][2]), s1[2],
+                            fma(quant<QP>((CT)hd[i][1]), s1[1], fma(quant<QP>((CT)hd[i][0]), s1[0], s2[i][q3])));
         }
       }
       // stage 3 for this type of third-direction factors, the 1-D rows read as wavefront-uniform
       // values from memory (scalar loads):
-      //   E[a3][b3] += sum_q3 Xt(a3, q3) (Yc(b3, q3) s2[b2][q3])
+      //   E[i][a3][b3] += sum_q3 Xt(a3, q3) (Yc(b3, q3) s2[i][q3])
       const double* bt = t1t + (t >> 1) * NT * 3;
       const double* bc = t1c + (t & 1) * NC * 3;
       if constexpr (LOWP) {
         // the stage-3 operands: s2 and the table products, each rounded
 #pragma unroll
-        for (int b2 = 0; b2 < NC; ++b2)
+        for (int i = 0; i < N2; ++i)
 #pragma unroll
-          for (int q = 0; q < 3; ++q) s2[b2][q] = quant<QP>(s2[b2][q]);
+          for (int q = 0; q < 3; ++q) s2[i][q] = quant<QP>(s2[i][q]);
 #pragma unroll
         for (int a3 = 0; a3 < NT; ++a3)
 #pragma unroll
@@ -374,24 +382,24 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
             const double* f3 = PT + ((t * NT + a3) * NC + b3) * 3;
             const CT k0 = quant<QP>((CT)f3[0]), k1 = quant<QP>((CT)f3[1]), k2 = quant<QP>((CT)f3[2]);
 #pragma unroll
-            for (int b2 = 0; b2 < NC; ++b2)
-              acc[b2][a3][b3] = fma(k2, s2[b2][2], fma(k1, s2[b2][1], fma(k0, s2[b2][0], acc[b2][a3][b3])));
+            for (int i = 0; i < N2; ++i)
+              acc[i][a3][b3] = fma(k2, s2[i][2], fma(k1, s2[i][1], fma(k0, s2[i][0], acc[i][a3][b3])));
           }
         continue;
       }
 #pragma unroll
       for (int b3 = 0; b3 < NC; ++b3) {
-        CT u[NC][3];
+        CT u[N2][3];
 #pragma unroll
-        for (int b2 = 0; b2 < NC; ++b2)
+        for (int i = 0; i < N2; ++i)
 #pragma unroll
-          for (int q = 0; q < 3; ++q) u[b2][q] = (CT)bc[b3 * 3 + q] * s2[b2][q];
+          for (int q = 0; q < 3; ++q) u[i][q] = (CT)bc[b3 * 3 + q] * s2[i][q];
 #pragma unroll
         for (int a3 = 0; a3 < NT; ++a3) {
           const CT k0 = (CT)bt[a3 * 3], k1 = (CT)bt[a3 * 3 + 1], k2 = (CT)bt[a3 * 3 + 2];
 #pragma unroll
-          for (int b2 = 0; b2 < NC; ++b2)
-            acc[b2][a3][b3] = fma(k2, u[b2][2], fma(k1, u[b2][1], fma(k0, u[b2][0], acc[b2][a3][b3])));
+          for (int i = 0; i < N2; ++i)
+            acc[i][a3][b3] = fma(k2, u[i][2], fma(k1, u[i][1], fma(k0, u[i][0], acc[i][a3][b3])));
         }
       }
     }
@@ -409,7 +417,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
 template <typename CT, int NT, int NC, int QP>
 static int launch_sumfact(const hm_problem& P, int i, int j, int64_t e0, int64_t e1, const SumfactPlan& SP,
                           void* out, hipStream_t st) {
-  constexpr int EPW = 64 / (NT * NC * NT);
+  constexpr int EPW = 64 / (NT * NC * (NC >= NT ? NC : NT));
   const size_t bytes = ((size_t)sf_table_words(P.nd_u, P.nd_m, SP.need_fields != 0) +
                         (size_t)SF_WPB * EPW * sf_element_words(SP.npair, P.nd_u, P.nd_m)) * sizeof(double);
   int limit = 64 * 1024, dev = 0;
