@@ -114,9 +114,10 @@ def algorithmic_bytes(blk, W):
 
     element kernel: the element matrices written (8 or 4 bytes an entry), the element vertex
     and dof tables and the vertex coordinates and dof values read (the fields only where the
-    block reads them); scatter: the element matrices read, one code byte an entry, the runs
-    (4 bytes an element row), the row tables (indptr, rptr 4 bytes, rowmode 1 byte a row) and
-    the values written."""
+    block reads them); scatter: the element matrices read, one code byte an entry, the row
+    tables (a 48-byte row header with the first eight runs where the element rows have more
+    than 8 entries; otherwise the runs, 4 bytes an element row, and indptr, rptr and rowmode,
+    9 bytes a row) and the values written."""
     pat = blk.pattern
     k = blk.kernel
     esize = 8 if blk.precision == "fp64" else 4
@@ -127,7 +128,8 @@ def algorithmic_bytes(blk, W):
     fields = not (blk.i == 2 and blk.j == 0 and not k.density.nonlinear)
     b_elem = (esize * nent + 4 * 8 * ne + 24 * W.mesh.nvert + 4 * ne * Vm.nd + 8 * Vm.ndof +
               ((4 * ne * Vu.nd + 16 * Vu.ndof) if fields else 0))
-    b_scatter = esize * nent + nent + 4 * ne * nd_t + 9 * pat.nrow + vsize * pat.nnz
+    tables = 48 * pat.nrow if nd_c > 8 else 4 * ne * nd_t + 9 * pat.nrow
+    b_scatter = esize * nent + nent + tables + vsize * pat.nnz
     return b_elem, b_scatter
 
 
