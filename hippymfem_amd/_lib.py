@@ -24,6 +24,7 @@ HM_ERR_ARG, HM_ERR_HIP, HM_ERR_ROW_LONG, HM_ERR_DUP, HM_ERR_NODIAG, HM_ERR_UNSUP
 EXPORTS = ("hm_last_error", "hm_version", "hm_element_matrices", "hm_consistent_block",
            "hm_element_vectors",
            "hm_element_hvp", "hm_pattern_build", "hm_pattern_place", "hm_pattern_repeats",
+           "hm_pattern_work_bytes", "hm_pattern_build_device", "hm_pattern_place_device",
            "hm_scatter_matrix", "hm_scatter_vector", "hm_csr_matvec")
 
 P = ctypes.c_void_p
@@ -77,6 +78,11 @@ def lib():
                                             P, ctypes.c_int, P, P]),
         "hm_pattern_repeats": (ctypes.c_int, [I64, ctypes.c_int, ctypes.c_int, P,
                                               ctypes.c_int, P]),
+        "hm_pattern_work_bytes": (I64, [I64]),
+        "hm_pattern_build_device": (ctypes.c_int, [P, P, I64, ctypes.c_int, ctypes.c_int, I64,
+                                                   I64, ctypes.c_int, P, P, P, P, P, P, P]),
+        "hm_pattern_place_device": (ctypes.c_int, [P, P, I64, ctypes.c_int, ctypes.c_int, I64,
+                                                   P, P, P, P]),
         "hm_scatter_matrix": (ctypes.c_int, [P, P, ctypes.c_int, I64, I64, P, ctypes.c_int,
                                              ctypes.c_int, P]),
         "hm_scatter_vector": (ctypes.c_int, [I64, ctypes.c_int, P, P, P, I64, I64, P, P,

@@ -51,6 +51,42 @@ def build_tables(tdofs, cdofs, nrow, ncol, diag_first, nthreads=0):
     return indptr, rptr, runs[:ne * nd_t], pos[:ne * nd_t * nd_c], int(nnz.value), int(max_row.value)
 
 
+def build_tables_device(tdofs, cdofs, nrow, ncol, diag_first, stream=None):
+    """The same build on the device (``hm_pattern_build_device``, then
+    ``hm_pattern_place_device``): a dict of device tensors ``indptr``, ``rptr``, ``runs``
+    (int32), ``pos`` (uint16 words held as int16), ``indices`` (int32), and ``nnz``,
+    ``max_row``, ``repeats``.  ``tdofs``/``cdofs`` are (ne, nd) int32 arrays, host or device."""
+    dev = _lib.require_device()
+    td = torch.as_tensor(np.ascontiguousarray(tdofs, dtype=np.int32)
+                         if isinstance(tdofs, np.ndarray) else tdofs).to(dev, torch.int32)
+    cd = torch.as_tensor(np.ascontiguousarray(cdofs, dtype=np.int32)
+                         if isinstance(cdofs, np.ndarray) else cdofs).to(dev, torch.int32)
+    td, cd = td.contiguous(), cd.contiguous()
+    if td.dim() != 2 or cd.dim() != 2 or td.shape[0] != cd.shape[0]:
+        raise ValueError("dof tables of shapes %s and %s" % (tuple(td.shape), tuple(cd.shape)))
+    ne, nd_t = td.shape
+    nd_c = cd.shape[1]
+    L = _lib.lib()
+    rptr = torch.empty(nrow + 1, dtype=torch.int32, device=dev)
+    indptr = torch.empty(nrow + 1, dtype=torch.int32, device=dev)
+    runs = torch.empty(max(ne * nd_t, 1), dtype=torch.int32, device=dev)
+    pos = torch.empty(max(ne * nd_t * nd_c, 1), dtype=torch.int16, device=dev)
+    work = torch.empty(int(L.hm_pattern_work_bytes(nrow)), dtype=torch.uint8, device=dev)
+    info = (_lib.I64 * 3)()
+    st = _lib.stream_handle(stream)
+    _lib.check(L.hm_pattern_build_device(_lib.ptr(td), _lib.ptr(cd), ne, nd_t, nd_c, nrow, ncol,
+                                         int(bool(diag_first)), _lib.ptr(rptr), _lib.ptr(runs),
+                                         _lib.ptr(pos), _lib.ptr(indptr), _lib.ptr(work), info,
+                                         st), "device pattern build")
+    nnz = int(info[0])
+    indices = torch.empty(max(nnz, 1), dtype=torch.int32, device=dev)
+    _lib.check(L.hm_pattern_place_device(_lib.ptr(td), _lib.ptr(cd), ne, nd_t, nd_c, nrow,
+                                         _lib.ptr(indptr), _lib.ptr(pos), _lib.ptr(indices), st),
+               "device pattern place")
+    return dict(indptr=indptr, rptr=rptr, runs=runs[:ne * nd_t], pos=pos[:ne * nd_t * nd_c],
+                indices=indices[:nnz], nnz=nnz, max_row=int(info[1]), repeats=bool(info[2]))
+
+
 class ScatterPattern:
     """CSR graph of the ``test x trial`` element-assembly map (``ScatterPattern``,
     ``pattern.py:375-1090``).
@@ -58,10 +94,14 @@ class ScatterPattern:
     ``indptr``/``indices`` int32 (hypre's ``HYPRE_Int``), ``nnz``, ``nent`` (element-matrix
     entries); ``slot`` (int32, the CSR place of entry ``k = (e nd_t + a) nd_c + b``) is made
     when first asked for.  Raises ``ValueError`` for a row of 2**16 entries or more, and
-    ``NotImplementedError`` for an element row that repeats a column."""
+    ``NotImplementedError`` for an element row that repeats a column.
+
+    ``build="device"`` builds on the GPU (:func:`build_tables_device`, the same arrays bit for
+    bit) and keeps the device arrays for the scatter plans; the host copies serve the host
+    paths."""
 
     def __init__(self, test_tables, trial_tables, test_space=None, trial_space=None,
-                 nthreads=0):
+                 nthreads=0, build="host"):
         if len(test_tables) != len(trial_tables):
             raise ValueError("test and trial tables must cover the same groups")
         if len(test_tables) != 1:
@@ -82,6 +122,20 @@ class ScatterPattern:
         if max(self.nrow, self.ncol) > (1 << 31) - 1:
             raise ValueError("%d rows and %d columns do not fit 32-bit local indices"
                              % (self.nrow, self.ncol))
+        self._plans = {}
+        if build == "device":
+            repeats = self._build_device()
+        elif build == "host":
+            repeats = self._build_host(nthreads)
+        else:
+            raise ValueError("build is 'host' or 'device', not %r" % (build,))
+        if repeats:
+            raise NotImplementedError("an element row repeats a column; the device scatter "
+                                      "adds one element row's entries at once")
+        self._slot = None
+        self._masked = {}
+
+    def _build_host(self, nthreads):
         (indptr, self.rptr, self.runs, self.pos, self.nnz,
          self.max_row) = build_tables(self.tdofs, self.cdofs, self.nrow, self.ncol,
                                       self.diag_first, nthreads)
@@ -98,12 +152,19 @@ class ScatterPattern:
         rep = _lib.I32(0)
         _lib.check(L.hm_pattern_repeats(self.ne, self.nd_t, self.nd_c, _lib.ptr(self.pos),
                                         _threads(), _lib.ctypes.byref(rep)), "pattern check")
-        if rep.value:
-            raise NotImplementedError("an element row repeats a column; the device scatter "
-                                      "adds one element row's entries at once")
-        self._slot = None
-        self._plans = {}
-        self._masked = {}
+        return bool(rep.value)
+
+    def _build_device(self):
+        d = build_tables_device(self.tdofs, self.cdofs, self.nrow, self.ncol, self.diag_first)
+        self.nnz, self.max_row = d["nnz"], d["max_row"]
+        self.indptr = d["indptr"].cpu().numpy()
+        self._indptr64 = self.indptr.astype(np.int64)
+        self.rptr = d["rptr"].cpu().numpy().astype(np.int64)
+        self.runs = d["runs"].cpu().numpy()
+        self.pos = d["pos"].cpu().numpy().view(np.uint16)
+        self.indices = d["indices"].cpu().numpy()
+        self._plans["base"] = dict(indptr=d["indptr"], rptr=d["rptr"], runs=d["runs"])
+        return d["repeats"]
 
     # ---------------------------------------------------------------- host arrays
     @property

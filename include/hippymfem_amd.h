@@ -134,6 +134,29 @@ int hm_pattern_place(const int32_t* cdofs, int64_t ne, int nd_t, int nd_c, int64
 int hm_pattern_repeats(int64_t ne, int nd_t, int nd_c, const uint16_t* pos, int nthreads,
                        int32_t* repeats);
 
+/* ---------------------------------------------------------------- the pattern (device)
+ * The same build from device dof tables, on the given stream, with the same arrays bit for
+ * bit (indptr and rptr as int32, hypre's diag-block index type): the device counterpart of
+ * build_tables (hippymfem/fem/patternbuild.py:209-253) and of the counting, ordering and
+ * placing kernels (hippymfem/fem/_patternkernels.py:110-183).  Integer atomics only; every
+ * list is sorted before use, so the result does not depend on scheduling.
+ *
+ * hm_pattern_work_bytes(nrow): size of the caller's device workspace for hm_pattern_build_device.
+ * hm_pattern_build_device: tdofs (ne, nd_t), cdofs (ne, nd_c) int32 device arrays.  Writes rptr
+ *   (nrow+1), runs (ne*nd_t), pos (ne*nd_t*nd_c) uint16 and indptr (nrow+1); info (host, 3
+ *   int64) = {nnz, longest row, 1 if an element row repeats a column}.  Synchronizes the
+ *   stream to read the totals.  Errors as hm_pattern_build; rows of more than 2048 entries
+ *   give HM_ERR_UNSUPPORTED (build those on the host).
+ * hm_pattern_place_device: indices (nnz) int32 from the build's indptr and pos. */
+int64_t hm_pattern_work_bytes(int64_t nrow);
+int hm_pattern_build_device(const int32_t* tdofs, const int32_t* cdofs, int64_t ne, int nd_t,
+                            int nd_c, int64_t nrow, int64_t ncol, int diag_first, int32_t* rptr,
+                            int32_t* runs, uint16_t* pos, int32_t* indptr, void* work,
+                            int64_t* info, void* stream);
+int hm_pattern_place_device(const int32_t* tdofs, const int32_t* cdofs, int64_t ne, int nd_t,
+                            int nd_c, int64_t nrow, const int32_t* indptr, const uint16_t* pos,
+                            int32_t* indices, void* stream);
+
 /* ------------------------------------------------------------------ the scatter
  * Device plan of one block's scatter: every CSR row sums the entries of its element rows
  * in ascending entry order, which is np.bincount's order (hippymfem/fem/pattern.py:788),

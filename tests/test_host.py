@@ -55,7 +55,7 @@ needs_lib = pytest.mark.skipif(not _built(), reason="native library not built")
 @needs_lib
 def test_library_exports_every_declared_symbol():
     header = open(os.path.join(ROOT, "include", "hippymfem_amd.h")).read()
-    declared = set(re.findall(r"^\s*(?:int|const char\*)\s+(hm_\w+)\s*\(", header, re.M))
+    declared = set(re.findall(r"^\s*(?:int|int64_t|const char\*)\s+(hm_\w+)\s*\(", header, re.M))
     assert declared == set(hm_lib.EXPORTS)
     L = ctypes.CDLL(hm_lib.LIB_PATH)
     for name in declared:
