@@ -29,11 +29,11 @@ struct Drop<uint16_t> {
   static constexpr unsigned v = 0xFFFFu;
 };
 
-// LPR lanes reduce one row (LPR a power of two up to 64, the element rows' entries taken LPR at
-// a time); a block of 256 threads holds 256/LPR row buffers.  The runs
-// of a row are taken eight at a time: their ids are read by one load and spread with a
-// lane broadcast, all eight runs' entries and codes are loaded before any is added (eight
-// loads in flight per lane), then added run after run in ascending order.
+// LPR lanes reduce one row (LPR a power of two up to 64, at least the element rows' entries: lane
+// b takes entry b of every run); a block of 256 threads holds 256/LPR row buffers.  The runs
+// of a row are taken eight at a time: their ids are read by one load and spread with a lane
+// broadcast, all eight runs' entries and codes are loaded before any is added (eight loads
+// in flight per lane), then added run after run in ascending order.
 template <typename CodeT, typename ET, typename OT, int LPR>
 __global__ void __launch_bounds__(256) k_scatter_rows(hm_scatter_plan S, const ET* __restrict__ E,
                                                       int64_t t0, int64_t t1, OT* __restrict__ data,
@@ -78,28 +78,27 @@ __global__ void __launch_bounds__(256) k_scatter_rows(hm_scatter_plan S, const E
     for (int32_t p = p0; p < p1; p += BATCH) {
       const int m = p1 - p < BATCH ? p1 - p : BATCH;
       const int32_t mine = (rh && p == p0) ? head : (lane < m ? S.runs[p + lane] : -1);
-      for (int b0 = 0; b0 < nd_c; b0 += LPR) {
-        const int b = b0 + lane;
-        double v[BATCH];
-        unsigned c[BATCH];
+      // the runs' entries (in their own type) and codes in flight before the first add
+      ET v[BATCH];
+      unsigned c[BATCH];
 #pragma unroll
-        for (int k = 0; k < BATCH; ++k) {
-          const int64_t t = __shfl(mine, k, LPR);
-          c[k] = DROP;
-          v[k] = 0.0;
-          if (k < m && b < nd_c && t >= t0 && t < t1) {
-            c[k] = codes[t * nd_c + b];
-            v[k] = (double)E[(t - t0) * nd_c + b];
-          }
+      for (int k = 0; k < BATCH; ++k) {
+        const int64_t t = __shfl(mine, k, LPR);
+        c[k] = DROP;
+        v[k] = (ET)0;
+        if (k < m && lane < nd_c && t >= t0 && t < t1) {
+          c[k] = codes[t * nd_c + lane];
+          v[k] = E[(t - t0) * nd_c + lane];
         }
-        // one LDS add a run: a wavefront's LDS operations complete in issue order, so the
-        // adds of run k land before those of run k + 1 and every sum keeps its order
-#pragma unroll
-        for (int k = 0; k < BATCH; ++k)
-          if (c[k] != DROP) atomicAdd(&buf[c[k]], v[k]);
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-        __builtin_amdgcn_wave_barrier();
       }
+      // one LDS add a run, run after run: a wavefront's LDS operations complete in issue
+      // order and the entries of one run have distinct places, so every sum takes its entries
+      // in ascending run order
+#pragma unroll
+      for (int k = 0; k < BATCH; ++k)
+        if (c[k] != DROP) atomicAdd(&buf[c[k]], (double)v[k]);
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      __builtin_amdgcn_wave_barrier();
     }
     for (int i = lane; i < len; i += LPR) data[lo + i] = (OT)buf[i];
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
@@ -155,15 +154,14 @@ static void launch_lpr(const hm_scatter_plan& S, const void* E, int64_t t0, int6
                      accumulate);
 }
 
-// Lanes a row (measured on MI355X at 64^3): as many as an element row has entries for rows of
-// up to 8 (more rows in flight, the reduction is latency-bound); 32 for FP64 element rows of
-// up to 32 entries, 16 for FP32 ones (half the bytes a load, two loads a run).
+// Lanes a row: the least power of two that holds an element row (measured on MI355X at 64^3:
+// the reduction is latency-bound, so more rows in flight win over wider rows).
 template <typename CodeT, typename ET, typename OT>
 static void launch_rows(const hm_scatter_plan& S, const void* E, int64_t t0, int64_t t1, void* data,
                         int accumulate, hipStream_t st) {
   if (S.nd_c <= 8)
     launch_lpr<CodeT, ET, OT, 8>(S, E, t0, t1, data, accumulate, st);
-  else if (S.nd_c <= 16 || (S.nd_c <= 32 && sizeof(ET) == 4))
+  else if (S.nd_c <= 16)
     launch_lpr<CodeT, ET, OT, 16>(S, E, t0, t1, data, accumulate, st);
   else if (S.nd_c <= 32)
     launch_lpr<CodeT, ET, OT, 32>(S, E, t0, t1, data, accumulate, st);
@@ -195,6 +193,8 @@ extern "C" int hm_scatter_matrix(const hm_scatter_plan* S, const void* E, int e_
                                  int64_t e1, void* data, int out_dtype, int accumulate, void* stream) {
   HM_REQUIRE(S != nullptr, "no scatter plan given");
   HM_REQUIRE(S->nrow >= 0 && S->nd_t > 0 && S->nd_c > 0, "bad scatter plan shape");
+  if (S->nd_c > HM_MAXND)
+    HM_FAIL(HM_ERR_UNSUPPORTED, "element rows of %d entries; the scatter takes at most %d", S->nd_c, HM_MAXND);
   HM_REQUIRE(e0 >= 0 && e0 <= e1 && e1 <= S->ne, "elements [%lld, %lld) outside [0, %lld)",
              (long long)e0, (long long)e1, (long long)S->ne);
   HM_REQUIRE(S->code_bytes == 1 || S->code_bytes == 2, "code width %d is neither 1 nor 2", S->code_bytes);

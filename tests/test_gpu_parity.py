@@ -132,6 +132,28 @@ def test_scatter_bitexact_with_bincount(gpu):
         assert np.array_equal(d, do), (i, j, np.abs(d - do).max())
 
 
+@pytest.mark.parametrize("out_dtype", [torch.float32, torch.float64])
+def test_scatter_fp32_entries_in_entry_order(gpu, out_dtype):
+    """FP32 element matrices of 27-entry rows (the scatter takes them in two passes of lanes):
+    every value is the FP64 sum of its entries in ascending entry order, rounded once, i.e.
+    np.bincount of the FP64 entries.  Entries spread over 2^+-40, so that any other order of
+    the sums shows in the result."""
+    from hippymfem_amd.elementbatch import group_tables
+    from hippymfem_amd.pattern import ScatterPattern
+
+    W = Workload(6)
+    g = W.groups[0]
+    rng = np.random.default_rng(11)
+    for T, C in ((W.Vu, W.Vu), (W.Vm, W.Vu)):
+        S = ScatterPattern([group_tables(T, g)], [group_tables(C, g)], T, C)
+        E = (rng.choice([-1.0, 1.0], S.nent) * rng.uniform(1, 2, S.nent) *
+             np.exp2(rng.integers(-40, 41, S.nent))).astype(np.float32)
+        got = S.data([torch.as_tensor(E).to(gpu)], dtype=out_dtype).cpu().numpy()
+        ref = np.bincount(S.slot, weights=E.astype(np.float64), minlength=S.nnz)
+        ref = ref.astype(np.float32) if out_dtype == torch.float32 else ref
+        assert got.dtype == ref.dtype and np.array_equal(got, ref), (T.ndof, C.ndof)
+
+
 def test_fused_chunks_equal_whole(gpu):
     W, _ = pair(6)
     whole = assemble_gpu(W, ADJOINT, STATE).data.cpu().numpy()
