@@ -3,7 +3,8 @@
 //
 // Every CSR row owns its sum.  Its entries come from its element rows (runs t = e*nd_t + a,
 // ascending, from the pattern build), each a contiguous run of nd_c entries of E, and the
-// place of every entry within the row (its code).  A row is reduced by one group of lanes
+// place of every entry within the row (its code, stored in run order so that a row's codes
+// lie together).  A row is reduced by one group of lanes
 // (half a wavefront when nd_c <= 32) into a row buffer in LDS, run after run in ascending
 // order, so every CSR value is the sum of its entries in ascending entry order: the order of
 // np.bincount, the reference's host scatter (hippymfem/fem/pattern.py:788).  The row buffer
@@ -87,7 +88,7 @@ __global__ void __launch_bounds__(256) k_scatter_rows(hm_scatter_plan S, const E
         c[k] = DROP;
         v[k] = (ET)0;
         if (k < m && lane < nd_c && t >= t0 && t < t1) {
-          c[k] = codes[t * nd_c + lane];
+          c[k] = codes[(int64_t)(p + k) * nd_c + lane];
           v[k] = E[(t - t0) * nd_c + lane];
         }
       }

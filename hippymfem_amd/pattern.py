@@ -227,6 +227,8 @@ class ScatterPattern:
                 c = c.reshape(self.ne, self.nd_t, self.nd_c)
                 c[dropped] = drop
                 c = c.reshape(-1)
+            # in run order: the codes of the run at position p at p * nd_c, a row's together
+            c = np.ascontiguousarray(c.reshape(-1, self.nd_c)[self.runs]).reshape(-1)
             ct = torch.from_numpy(c.view(np.uint8) if not wide else c.view(np.int16)).to(dev)
             codes = self._plans[codes_key] = (col_mask, ct)
         rowmode = None

@@ -164,7 +164,9 @@ int hm_pattern_place_device(const int32_t* tdofs, const int32_t* cdofs, int64_t 
  * row, 2 a zero row with `diag` at its first (diagonal) place; an entry whose code is
  * all ones (0xFF / 0xFFFF) is dropped (an eliminated column).
  *   indptr (nrow+1) int32, rptr (nrow+1) int32, runs (ne*nd_t) int32,
- *   codes (ne*nd_t*nd_c) uint8 (code_bytes 1) or uint16 (2), rowmode (nrow) int8 or NULL. */
+ *   codes (ne*nd_t*nd_c) uint8 (code_bytes 1) or uint16 (2): the place within its row of
+ *   entry b of the element row at run position p (runs[p]) is codes[p*nd_c + b], so that a
+ *   row's codes lie together; rowmode (nrow) int8 or NULL. */
 typedef struct hm_scatter_plan {
   int64_t nrow, nnz, ne;
   int32_t nd_t, nd_c, code_bytes, max_row;
