@@ -114,10 +114,16 @@ __host__ __device__ inline int sf_table_words(int nd_u, int nd_m, bool fields) {
   return 27 * 24 + 27 * nd_m + (fields ? 27 * nd_u * 4 : 0);
 }
 
+// Wavefronts a SIMD: four for the single-precision Q2 x Q2 forms (which fit 128 registers),
+// three for the others (168 registers; measured faster than two wavefronts with more).
+template <typename CT, int NT, int NC>
+constexpr int sf_waves() {
+  return (sizeof(CT) == 4 && NT == 3 && NC == 3) ? 4 : 3;
+}
+
 template <typename CT, int NT, int NC, int QP>
-// Three wavefronts a SIMD: the register budget that gives (a few registers spill on the
-// FP64 Q2 forms; measured faster than two wavefronts without spills).
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) k_sumfact(const hm_problem P, int s, int j, int64_t e0, int64_t e1,
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(sf_waves<CT, NT, NC>(), sf_waves<CT, NT, NC>()))) k_sumfact(const hm_problem P, int s, int j, int64_t e0, int64_t e1,
                                                  SumfactPlan SP, const double* __restrict__ t1t,
                                                  const double* __restrict__ t1c, CT* __restrict__ out) {
   // Lane task (a1, b1, x2): x2 = b2 with the a2 loop in registers where NC >= NT (for a fixed
@@ -158,21 +164,37 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
     }
   }
   __syncthreads();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int slot = lane / LPE, r = lane % LPE;
-  const bool in_use = slot < EPW;
-  const int a1 = r % NT, b1 = (r / NT) % NC, x2 = r / (NT * NC);
+  const int wave = threadIdx.x >> 6;
+  // The lane's fixed quantities (its element slot, task and LDS addresses).  The Q2 x Q2 forms
+  // derive them again every iteration from a value the compiler cannot see through: kept live
+  // across the loop they took registers that the FP64 form then spilled to scratch memory and
+  // reloaded every iteration through the vector memory path the element-matrix stores need,
+  // and that the single-precision forms need to fit four wavefronts a SIMD.
+  constexpr bool REDERIVE = NT == 3 && NC == 3;
+  int lane, slot, r, a1, b1, x2;
+  bool in_use;
+  double *Wl, *Xl, *Ul, *Pl, *Ml;
+  auto derive = [&](int lo) {
+    lane = lo & 63;
+    slot = lane / LPE;
+    r = lane % LPE;
+    in_use = slot < EPW;
+    a1 = r % NT;
+    b1 = (r / NT) % NC;
+    x2 = r / (NT * NC);
+    Wl = lds + sf_table_words(nu, nm, fields) +
+         (size_t)(wave * EPW + (in_use ? slot : 0)) * sf_element_words(npair, nu, nm);
+    Xl = Wl + npair * 27;
+    Ul = Xl + 24;
+    Pl = Ul + nu;
+    Ml = Pl + nu;
+  };
+  derive((int)threadIdx.x);
   // the (a2, b2) of register index i
   auto a2_of = [&](int i) { return BMAP ? i : x2; };
   auto b2_of = [&](int i) { return BMAP ? x2 : i; };
   const int32_t* lt = LX;
   const int32_t* lc = LX + nd_t;
-  double* Wl = lds + sf_table_words(nu, nm, fields) +
-               (size_t)(wave * EPW + (in_use ? slot : 0)) * sf_element_words(npair, nu, nm);
-  double* Xl = Wl + npair * 27;
-  double* Ul = Xl + 24;
-  double* Pl = Ul + nu;
-  double* Ml = Pl + nu;
   const int64_t ne = e1 - e0;
   const int64_t stride = (int64_t)gridDim.x * SF_WPB * EPW;
   // The next element's data travels in two steps, so that no wait falls on the stores of an
@@ -235,6 +257,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
   for (; base < ne; base += stride) {
     // the table reads stay inside the loop (hoisted, the products would take the registers)
     asm volatile("" ::: "memory");
+    if constexpr (REDERIVE) {
+      int lo = (int)threadIdx.x;
+      asm volatile("" : "+v"(lo));
+      derive(lo);
+    }
     const int64_t el = base + slot;
     const bool act = in_use && el < ne;
     // 1. the element's vertex coordinates and dof values into LDS; the next element's ids
