@@ -11,7 +11,11 @@ is the request loop around that call:
 * meshes, patterns, elimination plans and the CSR index arrays of every (n, order) are built
   once and stay resident (``ModelCache``);
 * each in-flight request owns a slot: a HIP stream of default priority and a set of device
-  buffers (element matrices, values, m, x, y) taken from its model's pool;
+  buffers (element matrices, values, m, x, y, its seed) taken from its model's pool;
+* a request's whole work (seed to the device, m and x drawn from it, element matrices,
+  scatter, y = A x, y to pinned host memory) is captured once per buffer set into a HIP
+  graph; a request then costs the host a seed write, one replay and an event, instead of a
+  launch per kernel (the small requests are bound by those launches);
 * requests are admitted in arrival order as soon as a slot is free and their buffers fit
   the HBM budget beside those in flight; admitted requests run concurrently on their
   streams, so their kernels overlap on the device, and a slot frees as soon as its request's
@@ -64,55 +68,60 @@ def poisson_requests(count, rate, seed=0, sizes=SIZES):
     return [Request(k, ns[k], orders[k], seeds[k], t[k]) for k in range(count)]
 
 
+def _seed_tensor(seed, device):
+    return torch.tensor([int(seed)], dtype=torch.int64, device=device)
+
+
 def device_field(n, seed, device, passes=3):
-    """The seeded parameter field on the ``(n+1)^3`` vertex lattice, drawn on the device:
-    N(0, 1) noise, ``passes`` of the 7-point average over the neighbours that exist, then
-    mean 0 and std 1 (the field of the benchmarks, drawn by the device generator)."""
-    g = torch.Generator(device=device)
-    g.manual_seed(int(seed))
-    m1 = n + 1
-    f = torch.randn((m1, m1, m1), generator=g, device=device, dtype=torch.float64)
-    cnt = torch.ones_like(f)
-    for ax in range(3):
-        cnt.narrow(ax, 1, m1 - 1).add_(1.0)
-        cnt.narrow(ax, 0, m1 - 1).add_(1.0)
-    for _ in range(passes):
-        acc = f.clone()
-        for ax in range(3):
-            acc.narrow(ax, 1, m1 - 1).add_(f.narrow(ax, 0, m1 - 1))
-            acc.narrow(ax, 0, m1 - 1).add_(f.narrow(ax, 1, m1 - 1))
-        f = acc / cnt
-    f = f.reshape(-1)
-    return (f - f.mean()) / f.std()
+    """The seeded parameter field on the ``(n+1)^3`` vertex lattice, drawn on the device
+    (``hm_serving_field``): N(0, 1) noise, ``passes`` of the 7-point average over the
+    neighbours that exist, then mean 0 and std 1 (the field of the benchmarks)."""
+    n1 = int(n) + 1
+    work = torch.empty(2 * n1 ** 3 + 2, dtype=torch.float64, device=device)
+    out = torch.empty(n1 ** 3, dtype=torch.float64, device=device)
+    sd = _seed_tensor(seed, device)
+    _lib.check(_lib.lib().hm_serving_field(_lib.ptr(sd), n1, int(passes), _lib.ptr(work), None,
+                                           _lib.ptr(out), _lib.stream_handle(None)), "field")
+    return out
 
 
 def device_direction(ndof, seed, device):
-    g = torch.Generator(device=device)
-    g.manual_seed(int(seed) + 1)
-    return torch.randn(ndof, generator=g, device=device, dtype=torch.float64)
+    """The seeded direction ``x``: ``ndof`` N(0, 1) values (``hm_serving_normal``, salt 1)."""
+    out = torch.empty(int(ndof), dtype=torch.float64, device=device)
+    sd = _seed_tensor(seed, device)
+    _lib.check(_lib.lib().hm_serving_normal(_lib.ptr(sd), 1, int(ndof), _lib.ptr(out),
+                                            _lib.stream_handle(None)), "direction")
+    return out
 
 
 class Buffers:
-    """The device memory of one in-flight request of a model."""
+    """The device memory of one in-flight request of a model, and the HIP graph of the
+    request's work on it (captured at the first use when the model runs graphs)."""
 
     def __init__(self, model):
         dev = model.device
         k = model.prob.kernel
         e_dtype = torch.float64 if model.precision == "fp64" else torch.float32
+        n1 = model.n + 1
         self.E = torch.empty((k.group.ne, k.nd[ADJOINT], k.nd[STATE]), dtype=e_dtype, device=dev)
         self.data = torch.empty(model.pattern.nnz, dtype=e_dtype, device=dev)
+        self.m = torch.empty(model.ndof_m, dtype=torch.float64, device=dev)
+        self.x = torch.empty(model.ndof, dtype=torch.float64, device=dev)
         self.y = torch.empty(model.ndof, dtype=torch.float64, device=dev)
+        self.work = torch.empty(2 * n1 ** 3 + 2, dtype=torch.float64, device=dev)
+        self.seed_dev = torch.zeros(1, dtype=torch.int64, device=dev)
+        self.seed_host = torch.zeros(1, dtype=torch.int64, pin_memory=True)
         self.y_host = torch.empty(model.ndof, dtype=torch.float64, pin_memory=True)
-        self.nbytes = (self.E.numel() * self.E.element_size() + self.data.numel() *
-                       self.data.element_size() + 8 * (2 * model.ndof + model.ndof_m))
+        self.graph = None
 
 
 class Model:
     """Everything of one (n, order) that stays resident: the mesh and space tables, the
     pattern and its elimination plan, the CSR index arrays, and a pool of request buffers."""
 
-    def __init__(self, n, order, precision, device):
+    def __init__(self, n, order, precision, device, graphs=False):
         self.n, self.order, self.precision, self.device = n, order, precision, device
+        self.graphs = bool(graphs)
         W = Workload(n, order=order)
         self.W = W
         self.prob = PDEVariationalProblem(W.spaces, W.density, W.batches, W.bc)
@@ -133,7 +142,8 @@ class Model:
         k = self.prob.kernel
         esz = 8 if precision == "fp64" else 4
         self.request_bytes = (esz * k.group.ne * k.nd[ADJOINT] * k.nd[STATE] + esz *
-                              self.pattern.nnz + 8 * (2 * self.ndof + self.ndof_m))
+                              self.pattern.nnz + 8 * (3 * self.ndof + self.ndof_m) +
+                              8 * (2 * (n + 1) ** 3 + 3))
 
     def take(self):
         return self.pool.pop() if self.pool else Buffers(self)
@@ -141,48 +151,90 @@ class Model:
     def give(self, b):
         self.pool.append(b)
 
+    def work(self, bufs):
+        """A request's work on the current stream, reading its seed from ``bufs.seed_host``:
+        the seed to the device, m and x drawn from it, element matrices, scatter, y = A x,
+        y to the host.  The same calls are captured into the buffers' graph."""
+        L = _lib.lib()
+        st = _lib.stream_handle(None)
+        bufs.seed_dev.copy_(bufs.seed_host, non_blocking=True)
+        _lib.check(L.hm_serving_field(_lib.ptr(bufs.seed_dev), self.n + 1, 3, _lib.ptr(bufs.work),
+                                      _lib.ptr(self.lattice_to_dof_m), _lib.ptr(bufs.m), st),
+                   "serving field")
+        _lib.check(L.hm_serving_normal(_lib.ptr(bufs.seed_dev), 1, self.ndof, _lib.ptr(bufs.x), st),
+                   "serving direction")
+        loc = [self.zeros_u, bufs.m, self.zeros_u]
+        self.prob.kernel.element_matrices(ADJOINT, STATE, loc, precision=self.precision,
+                                          out=bufs.E)
+        self.pattern.data([bufs.E], plan=self.dplan, out=bufs.data)
+        code = _lib.FP64 if bufs.data.dtype == torch.float64 else _lib.FP32
+        _lib.check(L.hm_csr_matvec(self.ndof, _lib.ptr(self.indptr), _lib.ptr(self.indices),
+                                   _lib.ptr(bufs.data), code, _lib.ptr(bufs.x), _lib.ptr(bufs.y),
+                                   st), "serving matvec")
+        bufs.y_host.copy_(bufs.y, non_blocking=True)
+
+    def capture(self, bufs):
+        """The buffers' graph: one eager run on a side stream, then the capture of the same
+        calls (kernels and copies only: no memset nodes)."""
+        side = torch.cuda.Stream(device=self.device)
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            self.work(bufs)
+        side.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=side, capture_error_mode="thread_local"):
+            self.work(bufs)
+        torch.cuda.current_stream().wait_stream(side)
+        bufs.graph = g
+
     def launch(self, req, bufs, stream):
-        """The request's work on its stream: draw m and x, element matrices, scatter,
-        y = A x, y to the host; returns the completion event."""
+        """The request's work on its stream (a graph replay when the model runs graphs);
+        returns the completion event.  The buffers' previous request has completed, so its
+        seed slot is free to write."""
+        bufs.seed_host[0] = req.seed
         with torch.cuda.stream(stream):
-            lat = device_field(self.n, req.seed, self.device)
-            m = torch.empty_like(lat)
-            m[self.lattice_to_dof_m] = lat
-            x = device_direction(self.ndof, req.seed, self.device)
-            loc = [self.zeros_u, m, self.zeros_u]
-            self.prob.kernel.element_matrices(ADJOINT, STATE, loc, precision=self.precision,
-                                              out=bufs.E, stream=stream)
-            self.pattern.data([bufs.E], plan=self.dplan, out=bufs.data, stream=stream)
-            code = _lib.FP64 if bufs.data.dtype == torch.float64 else _lib.FP32
-            _lib.check(_lib.lib().hm_csr_matvec(self.ndof, _lib.ptr(self.indptr),
-                                                _lib.ptr(self.indices), _lib.ptr(bufs.data),
-                                                code, _lib.ptr(x), _lib.ptr(bufs.y),
-                                                _lib.stream_handle(stream)), "serving matvec")
-            bufs.y_host.copy_(bufs.y, non_blocking=True)
+            if self.graphs:
+                if bufs.graph is None:
+                    self.capture(bufs)
+                bufs.graph.replay()
+            else:
+                self.work(bufs)
             ev = torch.cuda.Event()
             ev.record(stream)
-            # the request's temporaries stay alive until the event fires
-            bufs.keep = (lat, m, x)
         return ev
 
 
 class ModelCache:
-    def __init__(self, precision="fp32", device=None):
+    """The resident models; ``graphs``: requests run as replays of captured HIP graphs (one
+    per buffer set), so a request costs the host a few calls instead of one per kernel."""
+
+    def __init__(self, precision="fp32", device=None, graphs=True):
         self.precision = precision
         self.device = device or _lib.require_device()
+        self.graphs = bool(graphs)
         self.models = {}
 
     def get(self, n, order):
         key = (int(n), int(order))
         m = self.models.get(key)
         if m is None:
-            m = self.models[key] = Model(key[0], key[1], self.precision, self.device)
+            m = self.models[key] = Model(key[0], key[1], self.precision, self.device,
+                                         graphs=self.graphs)
         return m
 
-    def warm(self, sizes=SIZES, orders=(1, 2)):
+    def warm(self, sizes=SIZES, orders=(1, 2), buffers=0):
+        """Builds the models and, with ``buffers``, that many buffer sets each (their graphs
+        captured), so that no request waits for a capture."""
         for n in sizes:
             for o in orders:
-                self.get(n, o)
+                m = self.get(n, o)
+                made = [m.take() for _ in range(int(buffers))]
+                for b in made:
+                    if m.graphs and b.graph is None:
+                        m.capture(b)
+                for b in made:
+                    m.give(b)
+        torch.cuda.synchronize()
 
 
 class ServingEngine:
@@ -235,7 +287,6 @@ class ServingEngine:
                     r.done = time.perf_counter() - t0
                     if keep_results:
                         r.y = r.bufs.y_host.clone()
-                    r.bufs.keep = None
                     r.model.give(r.bufs)
                     used -= r.model.request_bytes
                     free.append(r.slot)
@@ -264,7 +315,6 @@ def reference_product(cache, req, precision="fp64"):
     ev = model.launch(req, bufs, stream)
     ev.synchronize()
     y = bufs.y_host.clone()
-    bufs.keep = None
     model.give(bufs)
     return y
 
@@ -302,7 +352,7 @@ def bench_sample(seconds=20.0, rate=None, slots=8, seed=0):
     default about half the measured capacity) for about ``seconds``; returns latency and
     throughput figures and the error against FP64."""
     cache = ModelCache("fp32")
-    cache.warm()
+    cache.warm(buffers=slots)
     eng = ServingEngine(cache, slots=slots)
     # capacity: a closed burst of mixed requests
     burst = poisson_requests(48, 1e6, seed=seed + 1)
@@ -334,9 +384,10 @@ def main(argv=None):
     ap.add_argument("--rates", default="0.5,0.8,0.95")
     ap.add_argument("--absolute", action="store_true")
     ap.add_argument("--slots", type=int, default=8)
+    ap.add_argument("--eager", action="store_true", help="launch every kernel (no graphs)")
     a = ap.parse_args(argv)
-    cache = ModelCache("fp32")
-    cache.warm()
+    cache = ModelCache("fp32", graphs=not a.eager)
+    cache.warm(buffers=a.slots)
     eng = ServingEngine(cache, slots=a.slots)
     burst = poisson_requests(96, 1e6, seed=7)
     for r in burst:

@@ -207,6 +207,21 @@ int hm_scatter_vector(int64_t n, int nd, const int32_t* rptr, const int32_t* run
 int hm_csr_matvec(int64_t nrow, const int32_t* indptr, const int32_t* indices,
                   const void* data, int val_dtype, const double* x, double* y, void* stream);
 
+/* ------------------------------------------------------------------ serving inputs
+ * No counterpart in the reference: the seeded inputs of the serving form (assemble-and-apply
+ * requests), drawn on the device from a seed held in device memory so that a captured HIP
+ * graph of a whole request draws new inputs at every replay.  Deterministic: the same seed
+ * gives the same bits in any launch shape.
+ *
+ * hm_serving_normal: out (n) fp64 N(0, 1) values of (*seed, salt).
+ * hm_serving_field: the benchmarks' parameter field on the n1^3 vertex lattice (x fastest):
+ *   N(0, 1) noise of (*seed, 0), `passes` 7-point averages over the neighbours that exist,
+ *   then mean 0 and unbiased standard deviation 1.  work (2 n1^3 + 2) fp64; out (n1^3) fp64
+ *   in lattice order, or out[lattice_to_dof[i]] when lattice_to_dof is not NULL. */
+int hm_serving_normal(const int64_t* seed, int64_t salt, int64_t n, double* out, void* stream);
+int hm_serving_field(const int64_t* seed, int n1, int passes, double* work,
+                     const int64_t* lattice_to_dof, double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

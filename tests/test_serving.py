@@ -87,3 +87,30 @@ def test_precision_form_error_and_admission(gpu):
     assert worst <= 1e-6
     with pytest.raises(ValueError):
         ServingEngine(cache, slots=2, hbm_budget=1024).run(reqs[:1])
+
+
+@pytest.mark.gpu
+def test_graph_replays_equal_eager(gpu):
+    """Every replay of a request's captured graph gives the bits of the same request run
+    eagerly, with other eager work and synchronizations between the replays."""
+    import torch
+
+    from hippymfem_amd.serving import ModelCache, Request, ServingEngine, device_field
+
+    eager = ModelCache("fp32", graphs=False)
+    graphs = ModelCache("fp32", graphs=True)
+    for n, order in ((8, 2), (16, 1)):
+        seeds = [31, 32, 31, 33]
+        ref = {}
+        for s in set(seeds):
+            r = Request(0, n, order, s)
+            ServingEngine(eager, slots=1).run([r])
+            ref[s] = r.y
+        eng = ServingEngine(graphs, slots=1)
+        for k, s in enumerate(seeds):
+            r = Request(k, n, order, s)
+            eng.run([r])
+            assert graphs.get(n, order).pool[0].graph is not None
+            assert torch.equal(r.y, ref[s]), (n, order, s)
+            device_field(n, 1000 + k, gpu)            # eager work between the replays
+            torch.cuda.synchronize()
