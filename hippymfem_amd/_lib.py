@@ -25,7 +25,7 @@ EXPORTS = ("hm_last_error", "hm_version", "hm_element_matrices", "hm_consistent_
            "hm_element_vectors",
            "hm_element_hvp", "hm_pattern_build", "hm_pattern_place", "hm_pattern_repeats",
            "hm_pattern_work_bytes", "hm_pattern_build_device", "hm_pattern_place_device",
-           "hm_scatter_matrix", "hm_scatter_vector", "hm_csr_matvec", "hm_serving_normal",
+           "hm_scatter_matrix", "hm_scatter_matrix_defect", "hm_scatter_vector", "hm_csr_matvec", "hm_serving_normal",
            "hm_serving_field")
 
 P = ctypes.c_void_p
@@ -86,6 +86,8 @@ def lib():
                                                    P, P, P, P]),
         "hm_scatter_matrix": (ctypes.c_int, [P, P, ctypes.c_int, I64, I64, P, ctypes.c_int,
                                              ctypes.c_int, P]),
+        "hm_scatter_matrix_defect": (ctypes.c_int, [P, P, ctypes.c_int, I64, I64, P,
+                                                    ctypes.c_int, P, P]),
         "hm_scatter_vector": (ctypes.c_int, [I64, ctypes.c_int, P, P, P, I64, I64, P, P,
                                              ctypes.c_int, P]),
         "hm_csr_matvec": (ctypes.c_int, [I64, P, P, P, ctypes.c_int, P, P, P]),

@@ -30,6 +30,30 @@ struct Drop<uint16_t> {
   static constexpr unsigned v = 0xFFFFu;
 };
 
+// Data-parallel moves between the lanes of a row of 16 (and from one row into the next):
+// a double moved as its two halves.  Lanes of rows left out of ROWS get 0.
+template <int CTRL, int ROWS = 0xF>
+__device__ inline double dpp_f64(double v) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, ROWS, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, ROWS, 0xF, false);
+  return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo);
+}
+
+// The sum of d over the LPR lanes of a group (groups aligned to LPR lanes), in a fixed order
+// on the vector units: pairs within quads, then the mirror pairs of 8 and of 16 lanes, then
+// the rows of 16 broadcast into the next.  Valid in the group's last lane.
+template <int LPR>
+__device__ inline double group_sum(double d) {
+  d += dpp_f64<0xB1>(d);  // quad_perm [1, 0, 3, 2]
+  d += dpp_f64<0x4E>(d);  // quad_perm [2, 3, 0, 1]
+  if constexpr (LPR >= 8) d += dpp_f64<0x141>(d);        // row_half_mirror
+  if constexpr (LPR >= 16) d += dpp_f64<0x140>(d);       // row_mirror
+  if constexpr (LPR >= 32) d += dpp_f64<0x142, 0xA>(d);  // row_bcast:15 into rows 1 and 3
+  if constexpr (LPR >= 64) d += dpp_f64<0x143, 0xC>(d);  // row_bcast:31 into rows 2 and 3
+  return d;
+}
+
 // LPR lanes reduce one row (LPR a power of two up to 64, at least the element rows' entries: lane
 // b takes entry b of every run); a block of 256 threads holds 256/LPR row buffers.  The runs
 // of a row are taken eight at a time: their ids are read by one load and spread with a lane
@@ -38,7 +62,7 @@ struct Drop<uint16_t> {
 template <typename CodeT, typename ET, typename OT, int LPR>
 __global__ void __launch_bounds__(256) k_scatter_rows(hm_scatter_plan S, const ET* __restrict__ E,
                                                       int64_t t0, int64_t t1, OT* __restrict__ data,
-                                                      int accumulate) {
+                                                      int accumulate, double* __restrict__ defect) {
   extern __shared__ double rowbuf[];
   constexpr int groups = 256 / LPR;
   constexpr int BATCH = 8;
@@ -71,6 +95,7 @@ __global__ void __launch_bounds__(256) k_scatter_rows(hm_scatter_plan S, const E
     }
     if (mode != 0) {
       for (int i = lane; i < len; i += LPR) data[lo + i] = (OT)((mode == 2 && i == 0) ? S.diag : 0.0);
+      if (defect && lane == 0) defect[r] = 0.0;
       continue;
     }
     for (int i = lane; i < len; i += LPR) buf[i] = accumulate ? (double)data[lo + i] : 0.0;
@@ -101,7 +126,20 @@ __global__ void __launch_bounds__(256) k_scatter_rows(hm_scatter_plan S, const E
       __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
       __builtin_amdgcn_wave_barrier();
     }
-    for (int i = lane; i < len; i += LPR) data[lo + i] = (OT)buf[i];
+    if (defect) {
+      // what the rounding to the values' type takes from the row's sum: each lane its
+      // places in order, then a fixed tree over the row's lanes
+      double d = 0.0;
+      for (int i = lane; i < len; i += LPR) {
+        const OT v = (OT)buf[i];
+        data[lo + i] = v;
+        d += buf[i] - (double)v;
+      }
+      d = group_sum<LPR>(d);
+      if (lane == LPR - 1) defect[r] = d;
+    } else {
+      for (int i = lane; i < len; i += LPR) data[lo + i] = (OT)buf[i];
+    }
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     __builtin_amdgcn_wave_barrier();
   }
@@ -146,41 +184,41 @@ __global__ void __launch_bounds__(256) k_csr_matvec(int64_t nrow, const int32_t*
 
 template <typename CodeT, typename ET, typename OT, int LPR>
 static void launch_lpr(const hm_scatter_plan& S, const void* E, int64_t t0, int64_t t1, void* data,
-                       int accumulate, hipStream_t st) {
+                       int accumulate, double* defect, hipStream_t st) {
   constexpr int groups = 256 / LPR;
   const int64_t nb = (S.nrow + groups - 1) / groups;
   const int grid = (int)(nb < 131072 ? (nb < 1 ? 1 : nb) : 131072);
   hipLaunchKernelGGL((k_scatter_rows<CodeT, ET, OT, LPR>), dim3(grid), dim3(256),
                      (size_t)groups * S.max_row * sizeof(double), st, S, (const ET*)E, t0, t1, (OT*)data,
-                     accumulate);
+                     accumulate, defect);
 }
 
 // Lanes a row: the least power of two that holds an element row (measured on MI355X at 64^3:
 // the reduction is latency-bound, so more rows in flight win over wider rows).
 template <typename CodeT, typename ET, typename OT>
 static void launch_rows(const hm_scatter_plan& S, const void* E, int64_t t0, int64_t t1, void* data,
-                        int accumulate, hipStream_t st) {
+                        int accumulate, double* defect, hipStream_t st) {
   if (S.nd_c <= 8)
-    launch_lpr<CodeT, ET, OT, 8>(S, E, t0, t1, data, accumulate, st);
+    launch_lpr<CodeT, ET, OT, 8>(S, E, t0, t1, data, accumulate, defect, st);
   else if (S.nd_c <= 16)
-    launch_lpr<CodeT, ET, OT, 16>(S, E, t0, t1, data, accumulate, st);
+    launch_lpr<CodeT, ET, OT, 16>(S, E, t0, t1, data, accumulate, defect, st);
   else if (S.nd_c <= 32)
-    launch_lpr<CodeT, ET, OT, 32>(S, E, t0, t1, data, accumulate, st);
+    launch_lpr<CodeT, ET, OT, 32>(S, E, t0, t1, data, accumulate, defect, st);
   else
-    launch_lpr<CodeT, ET, OT, 64>(S, E, t0, t1, data, accumulate, st);
+    launch_lpr<CodeT, ET, OT, 64>(S, E, t0, t1, data, accumulate, defect, st);
 }
 
 template <typename CodeT>
 static int dispatch_types(const hm_scatter_plan& S, const void* E, int e_dtype, int64_t t0, int64_t t1,
-                          void* data, int out_dtype, int accumulate, hipStream_t st) {
+                          void* data, int out_dtype, int accumulate, double* defect, hipStream_t st) {
   if (e_dtype == HM_FP64 && out_dtype == HM_FP64)
-    launch_rows<CodeT, double, double>(S, E, t0, t1, data, accumulate, st);
+    launch_rows<CodeT, double, double>(S, E, t0, t1, data, accumulate, defect, st);
   else if (e_dtype == HM_FP32 && out_dtype == HM_FP32)
-    launch_rows<CodeT, float, float>(S, E, t0, t1, data, accumulate, st);
+    launch_rows<CodeT, float, float>(S, E, t0, t1, data, accumulate, defect, st);
   else if (e_dtype == HM_FP32 && out_dtype == HM_FP64)
-    launch_rows<CodeT, float, double>(S, E, t0, t1, data, accumulate, st);
+    launch_rows<CodeT, float, double>(S, E, t0, t1, data, accumulate, defect, st);
   else if (e_dtype == HM_FP64 && out_dtype == HM_FP32)
-    launch_rows<CodeT, double, float>(S, E, t0, t1, data, accumulate, st);
+    launch_rows<CodeT, double, float>(S, E, t0, t1, data, accumulate, defect, st);
   else
     HM_FAIL(HM_ERR_ARG, "element dtype %d / value dtype %d not supported", e_dtype, out_dtype);
   return HM_OK;
@@ -190,8 +228,10 @@ static int dispatch_types(const hm_scatter_plan& S, const void* E, int e_dtype, 
 
 using namespace hm;
 
-extern "C" int hm_scatter_matrix(const hm_scatter_plan* S, const void* E, int e_dtype, int64_t e0,
-                                 int64_t e1, void* data, int out_dtype, int accumulate, void* stream) {
+namespace {
+
+static int scatter_matrix(const hm_scatter_plan* S, const void* E, int e_dtype, int64_t e0, int64_t e1, void* data,
+                          int out_dtype, int accumulate, double* defect, void* stream) {
   HM_REQUIRE(S != nullptr, "no scatter plan given");
   HM_REQUIRE(S->nrow >= 0 && S->nd_t > 0 && S->nd_c > 0, "bad scatter plan shape");
   if (S->nd_c > HM_MAXND)
@@ -205,10 +245,24 @@ extern "C" int hm_scatter_matrix(const hm_scatter_plan* S, const void* E, int e_
   const int64_t t0 = e0 * S->nd_t, t1 = e1 * S->nd_t;
   hipStream_t st = (hipStream_t)stream;
   int rc = S->code_bytes == 1
-               ? dispatch_types<uint8_t>(*S, E, e_dtype, t0, t1, data, out_dtype, accumulate, st)
-               : dispatch_types<uint16_t>(*S, E, e_dtype, t0, t1, data, out_dtype, accumulate, st);
+               ? dispatch_types<uint8_t>(*S, E, e_dtype, t0, t1, data, out_dtype, accumulate, defect, st)
+               : dispatch_types<uint16_t>(*S, E, e_dtype, t0, t1, data, out_dtype, accumulate, defect, st);
   if (rc) return rc;
   return hip_check("hm_scatter_matrix");
+}
+
+}  // namespace
+
+extern "C" int hm_scatter_matrix(const hm_scatter_plan* S, const void* E, int e_dtype, int64_t e0,
+                                 int64_t e1, void* data, int out_dtype, int accumulate, void* stream) {
+  return scatter_matrix(S, E, e_dtype, e0, e1, data, out_dtype, accumulate, nullptr, stream);
+}
+
+extern "C" int hm_scatter_matrix_defect(const hm_scatter_plan* S, const void* E, int e_dtype, int64_t e0,
+                                        int64_t e1, void* data, int out_dtype, double* row_defect,
+                                        void* stream) {
+  HM_REQUIRE(row_defect != nullptr, "no row defect array given");
+  return scatter_matrix(S, E, e_dtype, e0, e1, data, out_dtype, 0, row_defect, stream);
 }
 
 extern "C" int hm_scatter_vector(int64_t n, int nd, const int32_t* rptr, const int32_t* runs,

@@ -25,6 +25,9 @@ class ParCSRMatrix:
         self.shape = (pattern.nrow, pattern.ncol)
         self.row_starts = np.array([0, pattern.nrow]) if row_starts is None else row_starts
         self.col_starts = np.array([0, pattern.ncol]) if col_starts is None else col_starts
+        #: for single-precision values: what their rounding took from each row's sum
+        #: (``RowDefect``, ``singlesolve.py:682-700``), or None
+        self.row_defect = None
 
     @property
     def indices(self):

@@ -280,12 +280,14 @@ class ScatterPattern:
 
     # ------------------------------------------------------------------ values
     def data(self, element_matrices, zero_slots=None, host=False, dtype=None, plan=None,
-             out=None, stream=None):
+             out=None, stream=None, row_defect=None):
         """CSR ``data`` for one batch of element matrices (``ScatterPattern.data``,
         ``pattern.py:749-791``): the entries of every slot summed in ascending entry order
         on the device, then ``zero_slots`` cleared.  ``plan`` is a folded plan of
         :meth:`device_plan` (then ``zero_slots`` is not needed).  ``dtype``: the values'
-        precision (that of the element matrices by default)."""
+        precision (that of the element matrices by default).  ``row_defect`` (a device
+        fp64 array of ``nrow``): also what the rounding of each row's FP64 sums to ``dtype``
+        takes from the row's sum (``RowDefect``, ``singlesolve.py:682-700``)."""
         if len(element_matrices) != len(self.shapes):
             raise ValueError("expected %d element arrays, got %d"
                              % (len(self.shapes), len(element_matrices)))
@@ -297,9 +299,19 @@ class ScatterPattern:
         if out is None:
             out = torch.empty(self.nnz, dtype=dtype, device=E.device)
         p = plan if plan is not None else self.device_plan()
-        _lib.check(_lib.lib().hm_scatter_matrix(
-            _lib.ctypes.byref(p["plan"]), _lib.ptr(E), _dtype_code(E.dtype), 0, self.ne,
-            _lib.ptr(out), _dtype_code(out.dtype), 0, _lib.stream_handle(stream)), "scatter")
+        if row_defect is not None:
+            if (row_defect.dtype != torch.float64 or row_defect.numel() != self.nrow
+                    or row_defect.device != out.device):
+                raise ValueError("row_defect must be a device float64 array of %d" % self.nrow)
+            _lib.check(_lib.lib().hm_scatter_matrix_defect(
+                _lib.ctypes.byref(p["plan"]), _lib.ptr(E), _dtype_code(E.dtype), 0, self.ne,
+                _lib.ptr(out), _dtype_code(out.dtype), _lib.ptr(row_defect),
+                _lib.stream_handle(stream)), "scatter")
+        else:
+            _lib.check(_lib.lib().hm_scatter_matrix(
+                _lib.ctypes.byref(p["plan"]), _lib.ptr(E), _dtype_code(E.dtype), 0, self.ne,
+                _lib.ptr(out), _dtype_code(out.dtype), 0, _lib.stream_handle(stream)),
+                "scatter")
         if zero_slots is not None and len(zero_slots):
             out.index_fill_(0, _index_tensor(zero_slots, out.device), 0.0)
         return out.cpu().numpy() if host else out

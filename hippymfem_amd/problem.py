@@ -36,10 +36,17 @@ def _symmetric(i, j):
 
 class BlockAssembly:
     """Resident buffers and plan of one block: ``assemble`` is one element-kernel launch
-    into ``E`` and one scatter launch into ``data``, both on one stream."""
+    into ``E`` and one scatter launch into ``data``, both on one stream.
+
+    With ``row_sums``, a square block with single-precision values keeps the defect of its
+    row sums beside them (``row_defect``, fp64: what the rounding of each row's FP64 sums
+    took from the row's sum), as the reference's single-precision matrices do for their
+    solver (``tdofassemble.SINGLE_ROW_SUMS``, ``tdofassemble.py:107-125``).  It costs the
+    scatter a reduction and a store a row (about 0.06 ms on A at 64^3), so it is asked for
+    rather than made by default here, where no solver reads it."""
 
     def __init__(self, kernel, i, j, test_ess=None, diag_policy="one", precision="fp64",
-                 value_dtype=None):
+                 value_dtype=None, row_sums=False):
         self.kernel, self.i, self.j = kernel, i, j
         Vi, Vj = kernel.spaces[i], kernel.spaces[j]
         self.plan = plan_block(Vi, Vj, kernel.batches.groups, test_ess=test_ess,
@@ -54,18 +61,24 @@ class BlockAssembly:
         ne = kernel.group.ne
         self.E = torch.empty((ne, kernel.nd[i], kernel.nd[j]), dtype=e_dtype, device=dev)
         self.data = torch.empty(self.pattern.nnz, dtype=value_dtype, device=dev)
+        self.row_defect = None
+        if row_sums and Vi is Vj and value_dtype != torch.float64:
+            self.row_defect = torch.empty(self.pattern.nrow, dtype=torch.float64, device=dev)
 
     def element_matrices(self, local_arrays, stream=None):
         return self.kernel.element_matrices(self.i, self.j, local_arrays, precision=self.precision,
                                             out=self.E, stream=stream)[0]
 
     def scatter(self, stream=None):
-        return self.pattern.data([self.E], plan=self.dplan, out=self.data, stream=stream)
+        return self.pattern.data([self.E], plan=self.dplan, out=self.data, stream=stream,
+                                 row_defect=self.row_defect)
 
     def assemble(self, local_arrays, stream=None):
         self.element_matrices(local_arrays, stream)
         self.scatter(stream)
-        return finish_block(self.plan, self.data)
+        A = finish_block(self.plan, self.data)
+        A.row_defect = self.row_defect
+        return A
 
 
 class PDEVariationalProblem:
@@ -87,14 +100,15 @@ class PDEVariationalProblem:
         return [x[STATE], x[PARAMETER], x[ADJOINT] if len(x) > ADJOINT and x[ADJOINT] is not None
                 else np.zeros(self.Vh[ADJOINT].ndof)]
 
-    def block_assembly(self, i, j, precision="fp64", value_dtype=None):
-        key = (i, j, precision, value_dtype)
+    def block_assembly(self, i, j, precision="fp64", value_dtype=None, row_sums=False):
+        key = (i, j, precision, value_dtype) + (("row_sums",) if row_sums else ())
         got = self._blocks.get(key)
         if got is None:
             rule = ELIMINATION[(i, j)]
             got = self._blocks[key] = BlockAssembly(
                 self.kernel, i, j, test_ess=self.bc0.ess_tdof if rule["test"] else None,
-                diag_policy=rule["diag_policy"], precision=precision, value_dtype=value_dtype)
+                diag_policy=rule["diag_policy"], precision=precision, value_dtype=value_dtype,
+                row_sums=row_sums)
         return got
 
     # ------------------------------------------------------------------ assembly

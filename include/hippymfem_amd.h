@@ -193,6 +193,17 @@ enum { HM_ROWHEAD = 12 };
 int hm_scatter_matrix(const hm_scatter_plan* S, const void* E, int e_dtype, int64_t e0,
                       int64_t e1, void* data, int out_dtype, int accumulate, void* stream);
 
+/* hm_scatter_matrix of a whole batch (data overwritten) that also writes row_defect[nrow]
+ * fp64: what the rounding of each row's FP64 sums to the values' type takes from the row's
+ * sum, sum_j (a_ij - fl(a_ij)) (0 for FP64 values and for eliminated rows), so that the
+ * values plus diag(row_defect) have the row sums of the FP64 sums.  Replaces the row defect
+ * of a single-precision matrix (hippymfem/algorithms/singlesolve.py:682-700, the
+ * SINGLE_ROW_SUMS default of hippymfem/fem/tdofassemble.py:107-125), which the reference
+ * finds from the rounded residues; here it is summed in FP64 beside the values. */
+int hm_scatter_matrix_defect(const hm_scatter_plan* S, const void* E, int e_dtype, int64_t e0,
+                             int64_t e1, void* data, int out_dtype, double* row_defect,
+                             void* stream);
+
 /* out[n] (+)= the element vectors V (e1-e0, nd) of [e0, e1) by dof, in entry order, then
  * out[r] = 0 where zero_rows[r] != 0 (zero_rows may be NULL).  rptr/runs as the plan's.
  * Replaces VectorPattern.scatter (hippymfem/fem/pattern.py:1186-1209) and zero_essential

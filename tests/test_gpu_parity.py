@@ -134,8 +134,8 @@ def test_scatter_bitexact_with_bincount(gpu):
 
 @pytest.mark.parametrize("out_dtype", [torch.float32, torch.float64])
 def test_scatter_fp32_entries_in_entry_order(gpu, out_dtype):
-    """FP32 element matrices of 27-entry rows (the scatter takes them in two passes of lanes):
-    every value is the FP64 sum of its entries in ascending entry order, rounded once, i.e.
+    """FP32 element matrices of 27-entry rows: every value is the FP64 sum of its entries in
+    ascending entry order, rounded once, i.e.
     np.bincount of the FP64 entries.  Entries spread over 2^+-40, so that any other order of
     the sums shows in the result."""
     from hippymfem_amd.elementbatch import group_tables
@@ -152,6 +152,47 @@ def test_scatter_fp32_entries_in_entry_order(gpu, out_dtype):
         ref = np.bincount(S.slot, weights=E.astype(np.float64), minlength=S.nnz)
         ref = ref.astype(np.float32) if out_dtype == torch.float32 else ref
         assert got.dtype == ref.dtype and np.array_equal(got, ref), (T.ndof, C.ndof)
+
+
+def test_row_defect_of_single_values(gpu):
+    """FP32 values of the square block A keep, when asked, the defect of their row sums:
+    the FP64 sums (np.bincount of the FP32 element matrices) minus their FP32 roundings,
+    summed over each row; values plus diag(defect) have the FP64 row sums.  Eliminated rows
+    have none, and FP64 or non-square blocks (or a block not asking) keep no defect."""
+    from hippymfem_amd.problem import PDEVariationalProblem
+
+    W = Workload(6, perturb=0.2)
+    prob = PDEVariationalProblem(W.spaces, W.density, W.batches, W.bc)
+    loc = [torch.as_tensor(v).to(gpu) for v in W.locals()]
+    blk = prob.block_assembly(ADJOINT, STATE, precision="fp32", row_sums=True)
+    A = blk.assemble(loc)
+    assert A.row_defect is not None and A.row_defect.dtype == torch.float64
+    S = blk.pattern
+    E = blk.E.cpu().numpy().astype(np.float64).reshape(-1)
+    v64 = np.bincount(S.slot, weights=E, minlength=S.nnz)
+    mask = O.ess_mask(W.Vu.ndof, W.ess)
+    v64[O.masked_slots(S.indptr, S.indices, mask, mask)] = 0.0
+    ip = S.indptr.astype(np.int64)
+    rows = np.flatnonzero(mask)
+    v64[ip[rows]] = 1.0
+    v32 = v64.astype(np.float32)
+    got = A.data.cpu().numpy()
+    assert np.array_equal(got, v32)
+    res = v64 - v32.astype(np.float64)
+    row = np.repeat(np.arange(S.nrow), np.diff(ip))
+    d_ref = np.bincount(row, weights=res, minlength=S.nrow)
+    scale = np.bincount(row, weights=np.abs(res), minlength=S.nrow)
+    d = A.row_defect.cpu().numpy()
+    assert np.all(np.abs(d - d_ref) <= 1e-12 * scale + 1e-300)
+    assert np.all(d[rows] == 0.0) and np.any(d != 0.0)
+    sums64 = np.bincount(row, weights=v64, minlength=S.nrow)
+    sums32 = np.bincount(row, weights=v32.astype(np.float64), minlength=S.nrow)
+    size = np.bincount(row, weights=np.abs(v64), minlength=S.nrow)
+    assert np.all(np.abs(sums32 + d - sums64) <= 1e-14 * size)
+    assert prob.block_assembly(ADJOINT, STATE, row_sums=True).assemble(loc).row_defect is None
+    assert prob.block_assembly(ADJOINT, PARAMETER, precision="fp32",
+                               row_sums=True).assemble(loc).row_defect is None
+    assert prob.block_assembly(ADJOINT, STATE, precision="fp32").assemble(loc).row_defect is None
 
 
 def test_fused_chunks_equal_whole(gpu):
