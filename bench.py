@@ -142,21 +142,30 @@ def _parallel_cpu(P, ip, slot, nnz, kill, workers):
 def timed_steps(blk, loc, steps, warmup, stream):
     """Warm-up, then ``steps`` assemblies bracketed by synchronizations; HIP events on the
     stream the kernels run on split every step into its element-kernel and scatter launches."""
+    import gc
+
     import torch
 
     for _ in range(warmup):
         blk.assemble(loc, stream=stream)
     torch.cuda.synchronize()
     ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
-    t0 = time.perf_counter()
-    for k in range(steps):
-        ev[k][0].record(stream)
-        blk.element_matrices(loc, stream=stream)
-        ev[k][1].record(stream)
-        blk.scatter(stream=stream)
-        ev[k][2].record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+    # no garbage collection pause inside the timed region (a full collection after building
+    # a block's plan takes tens of milliseconds of host time)
+    gc.collect()
+    gc.disable()
+    try:
+        t0 = time.perf_counter()
+        for k in range(steps):
+            ev[k][0].record(stream)
+            blk.element_matrices(loc, stream=stream)
+            ev[k][1].record(stream)
+            blk.scatter(stream=stream)
+            ev[k][2].record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+    finally:
+        gc.enable()
     tk = float(np.mean([ev[k][0].elapsed_time(ev[k][1]) for k in range(steps)])) * 1e-3
     ts = float(np.mean([ev[k][1].elapsed_time(ev[k][2]) for k in range(steps)])) * 1e-3
     return wall / steps, tk, ts

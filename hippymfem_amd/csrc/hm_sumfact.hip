@@ -71,12 +71,16 @@ __host__ __device__ inline int sf_element_words(int npair, int nd_u, int nd_m) {
   return (w + 1) & ~1;
 }
 
-// The point state of point q from LDS: the element's vertex coordinates X, dof values U, M, P,
-// and the tables stored point-minor (entry (row, q) at row * 27 + q: lanes reading their own
-// points hit consecutive words) -- TX[(v * 3 + k)][q], TM[a][q], TU: N[a][q] then G[(a * 3 + k)][q].
+// The point state of point q from LDS: the element's vertex coordinates X, dof values M, and
+// U, P in tensor order (a1 + n1 (a2 + n1 a3)); the tables stored point-minor (entry (row, q)
+// at row * 27 + q: lanes reading their own points hit consecutive words) -- TX[(v * 3 + k)][q],
+// TM[a][q] -- and the state space's 1-D basis T1U[2][n1][3] (values, then derivatives).
+// The fields u, p and their reference gradients are summed one direction at a time: for each
+// (a2, a3) the sums over a1 of u and p against the value and the derivative of the 1-D basis
+// at q1, then those times the 1-D factors at q2 and q3.
 __device__ inline void sf_point_state(const hm_problem& P, int q, const double* X, const double* U,
                                       const double* M, const double* Pv, bool fields, const double* TX,
-                                      const double* TM, const double* TU, PointState& S) {
+                                      const double* TM, const double* T1U, PointState& S) {
   double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
 #pragma unroll 2
   for (int v = 0; v < 8; ++v)
@@ -87,19 +91,39 @@ __device__ inline void sf_point_state(const hm_problem& P, int q, const double* 
       J[1][k] = fma(X[v * 3 + 1], g, J[1][k]);
       J[2][k] = fma(X[v * 3 + 2], g, J[2][k]);
     }
-  const int nu = P.nd_u, nm = P.nd_m;
+  const int nm = P.nd_m;
   double uq = 0, mq = 0, ru[3] = {0, 0, 0}, rp[3] = {0, 0, 0};
   if (fields) {
-    const double* G = TU + 27 * nu;
-#pragma unroll 3
-    for (int a = 0; a < nu; ++a) {
-      const double ua = U[a], pa = Pv[a];
-      uq = fma(TU[a * 27 + q], ua, uq);
+    const int n1 = P.n1_u;
+    const int q1 = q % 3, q2 = (q / 3) % 3, q3 = q / 9;
+    // the 1-D factors are read from LDS as they are used (few registers: the point phase
+    // shares the kernel's register budget with the contraction)
+#pragma unroll 1
+    for (int a3 = 0; a3 < n1; ++a3) {
+      const double l3 = T1U[a3 * 3 + q3], d3 = T1U[(n1 + a3) * 3 + q3];
+#pragma unroll 1
+      for (int a2 = 0; a2 < n1; ++a2) {
+        const double l2 = T1U[a2 * 3 + q2], d2 = T1U[(n1 + a2) * 3 + q2];
+        const double* u = U + n1 * (a2 + n1 * a3);
+        const double* pv = Pv + n1 * (a2 + n1 * a3);
+        double svu = 0, sdu = 0, svp = 0, sdp = 0;
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const double g = G[(a * 3 + k) * 27 + q];
-        ru[k] = fma(g, ua, ru[k]);
-        rp[k] = fma(g, pa, rp[k]);
+        for (int a1 = 0; a1 < 3; ++a1) {
+          if (a1 >= n1) break;
+          const double l1 = T1U[a1 * 3 + q1], d1 = T1U[(n1 + a1) * 3 + q1];
+          svu = fma(l1, u[a1], svu);
+          sdu = fma(d1, u[a1], sdu);
+          svp = fma(l1, pv[a1], svp);
+          sdp = fma(d1, pv[a1], sdp);
+        }
+        const double f0 = l2 * l3, f1 = d2 * l3, f2 = l2 * d3;
+        uq = fma(f0, svu, uq);
+        ru[0] = fma(f0, sdu, ru[0]);
+        ru[1] = fma(f1, svu, ru[1]);
+        ru[2] = fma(f2, svu, ru[2]);
+        rp[0] = fma(f0, sdp, rp[0]);
+        rp[1] = fma(f1, svp, rp[1]);
+        rp[2] = fma(f2, svp, rp[2]);
       }
     }
   }
@@ -108,14 +132,14 @@ __device__ inline void sf_point_state(const hm_problem& P, int q, const double* 
 }
 
 // LDS words of the tables every element reads: the trilinear map's gradients [27][8][3], the
-// parameter basis [27][nd_m] and, when the fields are read, the state basis and its gradients
-// [27][nd_u] + [27][nd_u][3].
+// parameter basis [27][nd_m] and, when the fields are read, the state space's 1-D basis
+// [2][n1_u][3] (room for n1_u up to 3).
 __host__ __device__ inline int sf_table_words(int nd_u, int nd_m, bool fields) {
-  return 27 * 24 + 27 * nd_m + (fields ? 27 * nd_u * 4 : 0);
+  (void)nd_u;
+  const int w = 27 * 24 + 27 * nd_m + (fields ? 18 : 0);
+  return (w + 1) & ~1;
 }
 
-// Wavefronts a SIMD: four for the single-precision Q2 x Q2 forms (which fit 128 registers),
-// three for the others (168 registers; measured faster than two wavefronts with more).
 template <typename CT, int NT, int NC>
 constexpr int sf_waves() {
   return (sizeof(CT) == 4 && NT == 3 && NC == 3) ? 4 : 3;
@@ -139,12 +163,13 @@ __attribute__((amdgpu_waves_per_eu(sf_waves<CT, NT, NC>(), sf_waves<CT, NT, NC>(
   // derivative flags: every factor of stages 1, 2 and 3
   __shared__ double PT[4 * NT * NC * 3];
   __shared__ int32_t LX[nd_t + nd_c];  // local dof of each tensor index, test then trial
+  __shared__ int32_t UT[32];            // tensor index of each local state dof
   extern __shared__ double lds[];
   const int npair = SP.npair, nu = P.nd_u, nm = P.nd_m;
   const bool fields = SP.need_fields != 0;
   double* TX = lds;
   double* TM = TX + 27 * 24;
-  double* TU = TM + 27 * nm;
+  double* T1U = TM + 27 * nm;
   {
     const int32_t* lxt = s == HM_PARAMETER ? P.lex_m : P.lex_u;
     const int32_t* lxc = j == HM_PARAMETER ? P.lex_m : P.lex_u;
@@ -158,9 +183,8 @@ __attribute__((amdgpu_waves_per_eu(sf_waves<CT, NT, NC>(), sf_waves<CT, NT, NC>(
     for (int i = threadIdx.x; i < 27 * 24; i += blockDim.x) TX[(i % 24) * 27 + i / 24] = P.tab_x[i];
     for (int i = threadIdx.x; i < 27 * nm; i += blockDim.x) TM[(i % nm) * 27 + i / nm] = P.tab_m[i];
     if (fields) {
-      for (int i = threadIdx.x; i < 27 * nu; i += blockDim.x) TU[(i % nu) * 27 + i / nu] = P.tab_u[i];
-      for (int i = threadIdx.x; i < 27 * nu * 3; i += blockDim.x)
-        TU[27 * nu + (i % (nu * 3)) * 27 + i / (nu * 3)] = P.tab_u[27 * nu + i];
+      for (int i = threadIdx.x; i < 2 * P.n1_u * 3; i += blockDim.x) T1U[i] = P.tab1_u[i];
+      for (int i = threadIdx.x; i < nu; i += blockDim.x) UT[P.lex_u[i]] = i;
     }
   }
   __syncthreads();
@@ -275,8 +299,8 @@ __attribute__((amdgpu_waves_per_eu(sf_waves<CT, NT, NC>(), sf_waves<CT, NT, NC>(
       for (int k = 0; k < NPF; ++k) {
         const int a = r + k * LPE;
         if (fields && a < nu) {
-          Ul[a] = fu[k];
-          Pl[a] = fp[k];
+          Ul[UT[a]] = fu[k];
+          Pl[UT[a]] = fp[k];
         }
         if (a < nm) Ml[a] = fm[k];
       }
@@ -300,7 +324,7 @@ __attribute__((amdgpu_waves_per_eu(sf_waves<CT, NT, NC>(), sf_waves<CT, NT, NC>(
     if (act)
       for (int q = r; q < 27; q += LPE) {
         PointState S;
-        sf_point_state(P, q, Xl, Ul, Ml, Pl, fields, TX, TM, TU, S);
+        sf_point_state(P, q, Xl, Ul, Ml, Pl, fields, TX, TM, T1U, S);
         double D[4][4];
         point_block(s, j, S, D);
         int p = 0;
@@ -538,6 +562,8 @@ int hm_element_matrices_sumfact(const hm_problem* P, int i, int j, int64_t e0, i
     if (SP.pr[p] >= 1 && SP.pc[p] >= 1 && SP.pr[p] != SP.pc[p]) SP.offdiag |= 1 << p;
   }
   SP.need_fields = !(i * 3 + j == 6 && !nl);  // the linear Jacobian reads m and the geometry only
+  if (SP.need_fields && (P->n1_u < 2 || P->n1_u > 3))
+    HM_FAIL(HM_ERR_UNSUPPORTED, "state fields of 1-D order outside 1..2 for the sum-factorized contraction");
   hipStream_t st = (hipStream_t)stream;
   switch (precision) {
     case HM_FP64: return dispatch_orders<double, HM_FP64>(*P, i, j, e0, e1, nt, nc, SP, out, st);

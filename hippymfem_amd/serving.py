@@ -27,6 +27,7 @@ depend on what else is in flight: the same bits alone and in the mix.
 """
 
 import collections
+import gc
 import time
 
 import numpy as np
@@ -254,6 +255,16 @@ class ServingEngine:
         for r in requests:
             if self.cache.get(r.n, r.order).request_bytes > self.budget:
                 raise ValueError("request %d needs more than the HBM budget" % r.rid)
+        # the objects made so far (models, plans, buffers) leave the collector's view, so
+        # that a collection during the loop walks only what the loop makes
+        gc.collect()
+        gc.freeze()
+        try:
+            return self._run(requests, keep_results, poll_sleep)
+        finally:
+            gc.unfreeze()
+
+    def _run(self, requests, keep_results, poll_sleep):
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         waiting = collections.deque()
         free = list(range(len(self.slots)))
