@@ -28,6 +28,36 @@ ELIMINATION = {
 }
 
 
+def require_finite_arrays(arrays, what):
+    """Raise ``RuntimeError`` if an element array has a non-finite entry
+    (``require_finite_arrays``, ``PDEVariationalProblem.py:1844-1866``): a parameter may
+    be finite while ``exp(m)`` overflows inside the density, so only the element arrays
+    show it.  Reduced on the device; one flag comes back."""
+    ok = True
+    for a in arrays:
+        if isinstance(a, torch.Tensor):
+            ok = ok and bool(torch.isfinite(a).all())
+        else:
+            ok = ok and bool(np.isfinite(np.asarray(a)).all())
+        if not ok:
+            break
+    if not ok:
+        raise RuntimeError("%s produced non-finite element values; the residual density "
+                           "overflowed at this point (a line search should reject it)" % what)
+    return arrays
+
+
+def require_finite(v, what):
+    """Raise ``RuntimeError`` if the vector ``v`` has a non-finite entry
+    (``require_finite``, ``PDEVariationalProblem.py:1868-1890``)."""
+    ok = (bool(torch.isfinite(v).all()) if isinstance(v, torch.Tensor)
+          else bool(np.isfinite(np.asarray(v)).all()))
+    if not ok:
+        raise RuntimeError("%s contains non-finite values; the step that produced it is too "
+                           "large for this problem (a line search should reject it)" % what)
+    return v
+
+
 def _symmetric(i, j):
     """Whether block (i, j) is eliminated on both sides where they share a space
     (``PDEVariationalProblem._symmetric``)."""
@@ -115,15 +145,25 @@ class PDEVariationalProblem:
     def residual(self, x, var=ADJOINT, ess=None):
         """``dR/d(var)`` at ``x`` (``_residual``, ``PDEVariationalProblem.py:423-446``)."""
         vecs = self.kernel.element_vectors(var, self._locals(x))
+        require_finite_arrays(vecs, "the residual")
         return assemble_vector_csr(self.Vh[var], self.batches.groups, vecs, self.nelem, ess=ess)
 
     def block(self, i, j, x, precision="fp64"):
         """Block ``(i, j)`` at ``x`` with its elimination (``_block``/``_domain_block``,
-        ``PDEVariationalProblem.py:448-525``)."""
-        return self.block_assembly(i, j, precision).assemble(self._locals(x))
+        ``PDEVariationalProblem.py:448-525``); ``RuntimeError`` when an element matrix is
+        not finite (the check of ``_domain_block``, ``:495-525``)."""
+        blk = self.block_assembly(i, j, precision)
+        A = blk.assemble(self._locals(x))
+        require_finite_arrays([blk.E], "block (%d, %d)" % (i, j))
+        return A
 
     def jacobian(self, x, precision="fp64"):
-        """The forward Jacobian ``A`` (``_jacobian``, ``PDEVariationalProblem.py:542-659``)."""
+        """The forward Jacobian ``A`` (``_jacobian``, ``PDEVariationalProblem.py:542-659``):
+        the parameter (and, for a nonlinear residual, the state) must be finite
+        (``:555-557``)."""
+        require_finite(x[PARAMETER], "the parameter")
+        if not self.is_fwd_linear:
+            require_finite(x[STATE], "the state")
         return self.block(ADJOINT, STATE, x, precision)
 
     def set_linearization_point(self, x, gauss_newton_approx=False, precision="fp64"):

@@ -195,6 +195,26 @@ def test_row_defect_of_single_values(gpu):
     assert prob.block_assembly(ADJOINT, STATE, precision="fp32").assemble(loc).row_defect is None
 
 
+def test_non_finite_raises(gpu):
+    """A parameter whose exp overflows inside the density gives non-finite element values:
+    the residual and the blocks raise RuntimeError, as the reference's callers do
+    (PDEVariationalProblem.py:1844-1890); a non-finite parameter is refused up front."""
+    from hippymfem_amd.problem import PDEVariationalProblem
+
+    W = Workload(3)
+    prob = PDEVariationalProblem(W.spaces, W.density, W.batches, W.bc)
+    x = [W.u.copy(), W.m.copy(), W.p.copy()]
+    prob.block(ADJOINT, STATE, x)                      # finite: no error
+    x[PARAMETER][:] = 800.0                            # exp(800) overflows
+    with pytest.raises(RuntimeError):
+        prob.block(ADJOINT, STATE, x)
+    with pytest.raises(RuntimeError):
+        prob.residual(x)
+    x[PARAMETER][0] = np.nan
+    with pytest.raises(RuntimeError):
+        prob.jacobian(x)
+
+
 def test_fused_chunks_equal_whole(gpu):
     W, _ = pair(6)
     whole = assemble_gpu(W, ADJOINT, STATE).data.cpu().numpy()
