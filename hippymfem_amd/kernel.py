@@ -98,8 +98,7 @@ class QuadratureKernel:
         code = _lib.PRECISIONS[precision]
         dtype = torch.float64 if code == _lib.FP64 else torch.float32
         if out is None:
-            out = torch.empty((b - a, self.nd[i], self.nd[j]), dtype=dtype,
-                              device=keep[0].device)
+            out = _result_array((b - a, self.nd[i], self.nd[j]), dtype, keep[0].device)
         _lib.check(_lib.lib().hm_element_matrices(
             _lib.ctypes.byref(P), int(i), int(j), int(a), int(b), code, _lib.ptr(out),
             _lib.stream_handle(stream)), "element matrices (%d, %d)" % (i, j))
@@ -172,6 +171,21 @@ class QuadratureKernel:
                                              _lib.stream_handle(stream)),
                    "element products (%d, %d)" % (i, j))
         return [out]
+
+
+def _result_array(shape, dtype, dev):
+    """The result of a batch, or ``MemoryError`` when it does not fit on the device (as the
+    reference's batch planner says when no chunk size fits, ``kernel.py:1084-1101``): the
+    kernels hold nothing per element beyond their result, so only a smaller batch helps."""
+    try:
+        return torch.empty(shape, dtype=dtype, device=dev)
+    except torch.cuda.OutOfMemoryError as exc:
+        need = int(np.prod(shape)) * torch.empty((), dtype=dtype).element_size()
+        free = torch.cuda.mem_get_info(dev)[0] if dev.type == "cuda" else 0
+        raise MemoryError(
+            "element batch of %d does not fit on %s: its result is %.2f GB against %.2f GB "
+            "free. Assemble it in chunks (element_matrix_chunks, or a callable for "
+            "assemble_matrix)." % (shape[0], dev, need / 2 ** 30, free / 2 ** 30)) from exc
 
 
 def _vec(a, dev):
