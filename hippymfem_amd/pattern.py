@@ -4,7 +4,8 @@
 :class:`ScatterPattern` is the value-independent part of a matrix assembly: the CSR graph
 (``indptr``, ``indices``; rows of a square pattern lead with their diagonal, the order hypre
 keeps), the element rows of every CSR row (``rptr``/``runs``) and every entry's place in its
-row (``pos``), from the host build of the native library.  On the device it keeps one
+row (``pos``), from the native library's pattern build (on the GPU when one is visible, else
+on the host's threads; both give the same arrays bit for bit).  On the device it keeps one
 scatter plan per elimination: the rows, the runs and a one-byte code per entry (two where a
 row is longer than 254 entries).  :class:`VectorPattern` is the dual vectors' counterpart.
 """
@@ -98,10 +99,12 @@ class ScatterPattern:
 
     ``build="device"`` builds on the GPU (:func:`build_tables_device`, the same arrays bit for
     bit) and keeps the device arrays for the scatter plans; the host copies serve the host
-    paths."""
+    paths.  ``"host"`` builds on the host's threads.  ``"auto"`` (the default) builds on the
+    GPU when one is visible, and on the host where the device build does not take the
+    pattern (rows of more than 2048 entries) or there is no GPU."""
 
     def __init__(self, test_tables, trial_tables, test_space=None, trial_space=None,
-                 nthreads=0, build="host"):
+                 nthreads=0, build="auto"):
         if len(test_tables) != len(trial_tables):
             raise ValueError("test and trial tables must cover the same groups")
         if len(test_tables) != 1:
@@ -123,12 +126,23 @@ class ScatterPattern:
             raise ValueError("%d rows and %d columns do not fit 32-bit local indices"
                              % (self.nrow, self.ncol))
         self._plans = {}
-        if build == "device":
+        if build == "auto":
+            build = "device" if torch.cuda.is_available() else "host"
+            if build == "device":
+                try:
+                    repeats = self._build_device()
+                except NotImplementedError:
+                    self._plans = {}
+                    build = "host"
+            if build == "host":
+                repeats = self._build_host(nthreads)
+        elif build == "device":
             repeats = self._build_device()
         elif build == "host":
             repeats = self._build_host(nthreads)
         else:
-            raise ValueError("build is 'host' or 'device', not %r" % (build,))
+            raise ValueError("build is 'auto', 'host' or 'device', not %r" % (build,))
+        self.build = build
         if repeats:
             raise NotImplementedError("an element row repeats a column; the device scatter "
                                       "adds one element row's entries at once")

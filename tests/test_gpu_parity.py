@@ -217,19 +217,14 @@ def test_non_finite_raises(gpu):
 
 def test_batch_that_does_not_fit_raises_memory_error(gpu):
     """An element batch whose result does not fit raises MemoryError (the reference's
-    message when no chunk fits, kernel.py:1096-1101); chunks of it still assemble."""
-    W, _ = pair(8)                      # a 3 MB result: a block of its own in the allocator
-    loc = [torch.as_tensor(v).to(gpu) for v in W.locals()]
-    W.kernel.element_matrices(ADJOINT, STATE, loc)          # tables on the device
-    torch.cuda.synchronize()
-    torch.cuda.empty_cache()                                # no cached block to reuse
-    torch.cuda.set_per_process_memory_fraction(1e-7, gpu)   # no room for a new result
-    try:
-        with pytest.raises(MemoryError):
-            W.kernel.element_matrices(ADJOINT, STATE, loc)
-    finally:
-        torch.cuda.set_per_process_memory_fraction(1.0, gpu)
-    assert W.kernel.element_matrices(ADJOINT, STATE, loc)[0].shape == (512, 27, 27)
+    message when no chunk fits, kernel.py:1096-1101): a result of 10^8 Q2 element matrices
+    (583 GB) on a 288 GB device; a batch that fits still comes out."""
+    from hippymfem_amd.kernel import _result_array
+
+    with pytest.raises(MemoryError):
+        _result_array((10 ** 8, 27, 27), torch.float64, gpu)
+    W, _ = pair(4)
+    assert W.kernel.element_matrices(ADJOINT, STATE, W.locals())[0].shape == (64, 27, 27)
 
 
 def test_fused_chunks_equal_whole(gpu):

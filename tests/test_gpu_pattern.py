@@ -114,9 +114,10 @@ def test_device_built_pattern_assembles_identically(gpu):
     W = Workload(64)
     g = W.groups[0]
     tu = group_tables(W.Vu, g)
-    host = ScatterPattern([tu], [tu], W.Vu, W.Vu)
+    host = ScatterPattern([tu], [tu], W.Vu, W.Vu, build="host")
     torch.cuda.synchronize()
     dev = ScatterPattern([tu], [tu], W.Vu, W.Vu, build="device")
+    assert ScatterPattern([tu], [tu], W.Vu, W.Vu).build == "device"    # the default on a GPU
     assert dev.nnz == host.nnz == 513 ** 3
     for name in ("indptr", "indices", "rptr", "runs", "pos"):
         assert np.array_equal(getattr(dev, name), getattr(host, name)), name
