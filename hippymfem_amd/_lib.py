@@ -21,11 +21,11 @@ HM_ERR_ARG, HM_ERR_HIP, HM_ERR_ROW_LONG, HM_ERR_DUP, HM_ERR_NODIAG, HM_ERR_UNSUP
     -1, -2, -3, -4, -5, -6)
 
 #: every symbol ``include/hippymfem_amd.h`` declares
-EXPORTS = ("hm_last_error", "hm_version", "hm_element_matrices", "hm_consistent_block",
-           "hm_element_vectors",
-           "hm_element_hvp", "hm_pattern_build", "hm_pattern_place", "hm_pattern_repeats",
-           "hm_pattern_work_bytes", "hm_pattern_build_device", "hm_pattern_place_device",
-           "hm_scatter_matrix", "hm_scatter_matrix_defect", "hm_scatter_vector", "hm_csr_matvec", "hm_serving_normal",
+EXPORTS = ("hm_last_error", "hm_version", "hm_element_matrices", "hm_element_matrices_pair",
+           "hm_consistent_block", "hm_element_vectors", "hm_element_hvp", "hm_pattern_build",
+           "hm_pattern_place", "hm_pattern_repeats", "hm_pattern_work_bytes",
+           "hm_pattern_build_device", "hm_pattern_place_device", "hm_scatter_matrix",
+           "hm_scatter_matrix_defect", "hm_scatter_vector", "hm_csr_matvec", "hm_serving_normal",
            "hm_serving_field")
 
 P = ctypes.c_void_p
@@ -69,6 +69,9 @@ def lib():
         "hm_version": (ctypes.c_int, []),
         "hm_element_matrices": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, I64, I64,
                                                ctypes.c_int, P, P]),
+        "hm_element_matrices_pair": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                    ctypes.c_int, I64, I64, ctypes.c_int, P, P,
+                                                    P]),
         "hm_consistent_block": (ctypes.c_int, [P, P, P, I64, ctypes.c_int, ctypes.c_int,
                                                ctypes.c_int, P, P]),
         "hm_element_vectors": (ctypes.c_int, [P, ctypes.c_int, I64, I64, P, P]),

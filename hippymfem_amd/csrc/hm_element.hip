@@ -302,6 +302,22 @@ extern "C" int hm_element_matrices(const hm_problem* P, int i, int j, int64_t e0
   return hip_check("hm_element_matrices");
 }
 
+extern "C" int hm_element_matrices_pair(const hm_problem* P, int i1, int j1, int i2, int j2, int64_t e0,
+                                        int64_t e1, int precision, void* out1, void* out2, void* stream) {
+  if (int rc = check_problem(P)) return rc;
+  HM_REQUIRE(i1 >= 0 && i1 < 3 && j1 >= 0 && j1 < 3 && i2 >= 0 && i2 < 3 && j2 >= 0 && j2 < 3,
+             "slots (%d, %d), (%d, %d) outside 0..2", i1, j1, i2, j2);
+  HM_REQUIRE(i1 * 3 + j1 != i2 * 3 + j2, "block (%d, %d) twice", i1, j1);
+  HM_REQUIRE(e0 >= 0 && e0 <= e1 && e1 <= P->ne, "elements [%lld, %lld) outside [0, %lld)",
+             (long long)e0, (long long)e1, (long long)P->ne);
+  HM_REQUIRE(out1 && out2 && out1 != out2, "two distinct outputs needed");
+  if (e1 == e0) return HM_OK;
+  const char* route = getenv("HM_CONTRACTION");
+  if (route && (route[0] == 'v' || route[0] == 'm'))
+    HM_FAIL(HM_ERR_UNSUPPORTED, "two blocks a pass on the sum-factorized route only");
+  return hm_element_matrices_sumfact_pair(P, i1, j1, i2, j2, e0, e1, precision, out1, out2, stream);
+}
+
 extern "C" int hm_consistent_block(const float* K, const double* Rj, const double* Ri, int64_t ne, int nd_i,
                                    int nd_j, int symmetric, double* out, void* stream) {
   HM_REQUIRE(ne >= 0 && nd_i > 0 && nd_j > 0 && nd_i <= HM_MAXND && nd_j <= HM_MAXND,

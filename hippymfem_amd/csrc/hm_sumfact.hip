@@ -35,6 +35,10 @@ struct SumfactPlan {
   int tmask[4];                        // pairs of each third-direction type (r == 3, c == 3)
   int offdiag;                         // the pairs of two different reference derivatives
   int need_fields;                     // the pointwise blocks read u or p
+  // a second block of the same shape from the same pass (two-block kernels): its slots, its
+  // pairs (after the first block's in pr/pc) and the pairs of each block
+  int s2, j2, kmask2;
+  int bmask[2];
 };
 
 // Pairs left out of an element's contraction, their share of every entry being below the
@@ -145,11 +149,16 @@ constexpr int sf_waves() {
   return (sizeof(CT) == 4 && NT == 3 && NC == 3) ? 4 : 3;
 }
 
-template <typename CT, int NT, int NC, int QP>
+// NB = 2: the kernel writes two blocks of one shape (test and trial orders) from one pass over
+// the elements: geometry, fields and the pointwise pairs of both blocks once, then the two
+// contractions one after the other (the first block's matrix is stored at once, the second's
+// as the single block's is).
+template <typename CT, int NT, int NC, int QP, int NB>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(sf_waves<CT, NT, NC>(), sf_waves<CT, NT, NC>()))) k_sumfact(const hm_problem P, int s, int j, int64_t e0, int64_t e1,
                                                  SumfactPlan SP, const double* __restrict__ t1t,
-                                                 const double* __restrict__ t1c, CT* __restrict__ out) {
+                                                 const double* __restrict__ t1c, CT* __restrict__ out,
+                                                 CT* __restrict__ out2) {
   // Lane task (a1, b1, x2): x2 = b2 with the a2 loop in registers where NC >= NT (for a fixed
   // entry index the lanes of one a1 then write NC^2 adjacent columns), x2 = a2 with the b2 loop
   // otherwise (fewer registers for the wide-by-narrow blocks).
@@ -259,12 +268,13 @@ __attribute__((amdgpu_waves_per_eu(sf_waves<CT, NT, NC>(), sf_waves<CT, NT, NC>(
   // the element matrix of the previous iteration (-1: none), written once the next element's
   // values have arrived, so the stores never hold up a wait
   constexpr bool LOWP = QP == HM_BF16 || QP == HM_FP16;
+  static_assert(NB == 1 || NB == 2, "one or two blocks a pass");
+  static_assert(NB == 1 || !LOWP, "two blocks a pass in FP64 or FP32 only");
   CT acc[N2][NT][NC];
   int64_t prev_el = -1;
   double prev_scale = 1.0;
-  auto store_prev = [&]() {
-    if (prev_el < 0) return;
-    CT* Eo = out + (size_t)prev_el * nd_t * nd_c;
+  auto store_acc = [&](CT* dst, int64_t e_, double scale) {
+    CT* Eo = dst + (size_t)e_ * nd_t * nd_c;
 #pragma unroll
     for (int a3 = 0; a3 < NT; ++a3)
 #pragma unroll
@@ -272,8 +282,12 @@ __attribute__((amdgpu_waves_per_eu(sf_waves<CT, NT, NC>(), sf_waves<CT, NT, NC>(
         CT* Er = Eo + lt[a1 + NT * (a2_of(i) + NT * a3)] * nd_c;
 #pragma unroll
         for (int b3 = 0; b3 < NC; ++b3)
-          Er[lc[b1 + NC * (b2_of(i) + NC * b3)]] = LOWP ? (CT)(acc[i][a3][b3] * prev_scale) : acc[i][a3][b3];
+          Er[lc[b1 + NC * (b2_of(i) + NC * b3)]] = LOWP ? (CT)(acc[i][a3][b3] * scale) : acc[i][a3][b3];
       }
+  };
+  auto store_prev = [&]() {
+    if (prev_el < 0) return;
+    store_acc(NB == 2 ? out2 : out, prev_el, prev_scale);
   };
   int64_t base = ((int64_t)blockIdx.x * SF_WPB + wave) * EPW;
   fetch_ids(base + slot);
@@ -335,6 +349,15 @@ __attribute__((amdgpu_waves_per_eu(sf_waves<CT, NT, NC>(), sf_waves<CT, NT, NC>(
             Wl[p * 27 + q] = v;
             ++p;
           }
+        if constexpr (NB == 2) {
+          point_block(SP.s2, SP.j2, S, D);
+#pragma unroll
+          for (int k = 0; k < 16; ++k)
+            if ((SP.kmask2 >> k) & 1) {
+              Wl[p * 27 + q] = D[k >> 2][k & 3];
+              ++p;
+            }
+        }
       }
     wave_sync();
     // 3. the pairs this element needs.  The low-precision forms also measure every pair
@@ -361,6 +384,9 @@ __attribute__((amdgpu_waves_per_eu(sf_waves<CT, NT, NC>(), sf_waves<CT, NT, NC>(
     // 4. lane = contraction task (a1, b1, x2).  The low-precision forms scale the element's
     // pairs by its largest pointwise weight (operands within [-1, 1]) and the result back.
     const double wscale = (LOWP && top > 0.0) ? 1.0 / top : 1.0;
+#pragma unroll 1
+    for (int blk = 0; blk < NB; ++blk) {
+    const uint32_t bm = NB == 2 ? (uint32_t)SP.bmask[blk] : ~0u;
 #pragma unroll
     for (int i = 0; i < N2; ++i)
 #pragma unroll
@@ -369,7 +395,7 @@ __attribute__((amdgpu_waves_per_eu(sf_waves<CT, NT, NC>(), sf_waves<CT, NT, NC>(
         for (int b3 = 0; b3 < NC; ++b3) acc[i][a3][b3] = 0;
 #pragma unroll 1
     for (int t = 0; t < 4; ++t) {
-      const uint32_t tm = live & (uint32_t)SP.tmask[t];
+      const uint32_t tm = live & (uint32_t)SP.tmask[t] & bm;
       if (!act || tm == 0) continue;
       CT s2[N2][3];
 #pragma unroll
@@ -454,6 +480,10 @@ __attribute__((amdgpu_waves_per_eu(sf_waves<CT, NT, NC>(), sf_waves<CT, NT, NC>(
         }
       }
     }
+    // the first of two blocks leaves at once (its accumulators are needed for the second)
+    if constexpr (NB == 2)
+      if (blk == 0 && act) store_acc(out, el, 1.0);
+    }
     // the next element's values in flight; this element's matrix leaves at the top of the
     // next iteration, after those values are waited for
     fetch_values(el + stride);
@@ -465,9 +495,9 @@ __attribute__((amdgpu_waves_per_eu(sf_waves<CT, NT, NC>(), sf_waves<CT, NT, NC>(
   store_prev();
 }
 
-template <typename CT, int NT, int NC, int QP>
+template <typename CT, int NT, int NC, int QP, int NB>
 static int launch_sumfact(const hm_problem& P, int i, int j, int64_t e0, int64_t e1, const SumfactPlan& SP,
-                          void* out, hipStream_t st) {
+                          void* out, void* out2, hipStream_t st) {
   constexpr int EPW = 64 / (NT * NC * (NC >= NT ? NC : NT));
   const size_t bytes = ((size_t)sf_table_words(P.nd_u, P.nd_m, SP.need_fields != 0) +
                         (size_t)SF_WPB * EPW * sf_element_words(SP.npair, P.nd_u, P.nd_m)) * sizeof(double);
@@ -479,14 +509,14 @@ static int launch_sumfact(const hm_problem& P, int i, int j, int64_t e0, int64_t
   if (bytes + 4 * NT * NC * 3 * sizeof(double) > (size_t)limit)
     HM_FAIL(HM_ERR_UNSUPPORTED, "the sum-factorized contraction needs %zu bytes of LDS", bytes);
   if (bytes > 48 * 1024) {
-    hipError_t err = hipFuncSetAttribute((const void*)k_sumfact<CT, NT, NC, QP>,
+    hipError_t err = hipFuncSetAttribute((const void*)k_sumfact<CT, NT, NC, QP, NB>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (err != hipSuccess) HM_FAIL(HM_ERR_HIP, "hipFuncSetAttribute: %s", hipGetErrorString(err));
   }
   // persistent: as many blocks as the device holds at once, each walking many elements (the
   // tables are loaded once a block and the next element's data is in flight meanwhile)
   int per_cu = 0, cus = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_sumfact<CT, NT, NC, QP>, 256, bytes) !=
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_sumfact<CT, NT, NC, QP, NB>, 256, bytes) !=
           hipSuccess ||
       per_cu < 1)
     per_cu = 1;
@@ -494,41 +524,25 @@ static int launch_sumfact(const hm_problem& P, int i, int j, int64_t e0, int64_t
   const int64_t groups = (e1 - e0 + SF_WPB * EPW - 1) / (SF_WPB * EPW);
   const int64_t full = (int64_t)per_cu * cus;
   const int grid = (int)(groups < full ? groups : full);
-  hipLaunchKernelGGL((k_sumfact<CT, NT, NC, QP>), dim3(grid), dim3(256), bytes, st, P, i, j, e0, e1, SP,
+  hipLaunchKernelGGL((k_sumfact<CT, NT, NC, QP, NB>), dim3(grid), dim3(256), bytes, st, P, i, j, e0, e1, SP,
                      (const double*)(i == HM_PARAMETER ? P.tab1_m : P.tab1_u),
-                     (const double*)(j == HM_PARAMETER ? P.tab1_m : P.tab1_u), (CT*)out);
+                     (const double*)(j == HM_PARAMETER ? P.tab1_m : P.tab1_u), (CT*)out, (CT*)out2);
   return hip_check("k_sumfact");
 }
 
-template <typename CT, int QP>
+template <typename CT, int QP, int NB = 1>
 static int dispatch_orders(const hm_problem& P, int i, int j, int64_t e0, int64_t e1, int nt, int nc,
-                           const SumfactPlan& SP, void* out, hipStream_t st) {
-  if (nt == 3 && nc == 3) return launch_sumfact<CT, 3, 3, QP>(P, i, j, e0, e1, SP, out, st);
-  if (nt == 3 && nc == 2) return launch_sumfact<CT, 3, 2, QP>(P, i, j, e0, e1, SP, out, st);
-  if (nt == 2 && nc == 3) return launch_sumfact<CT, 2, 3, QP>(P, i, j, e0, e1, SP, out, st);
-  return launch_sumfact<CT, 2, 2, QP>(P, i, j, e0, e1, SP, out, st);
+                           const SumfactPlan& SP, void* out, hipStream_t st, void* out2 = nullptr) {
+  if (nt == 3 && nc == 3) return launch_sumfact<CT, 3, 3, QP, NB>(P, i, j, e0, e1, SP, out, out2, st);
+  if (nt == 3 && nc == 2) return launch_sumfact<CT, 3, 2, QP, NB>(P, i, j, e0, e1, SP, out, out2, st);
+  if (nt == 2 && nc == 3) return launch_sumfact<CT, 2, 3, QP, NB>(P, i, j, e0, e1, SP, out, out2, st);
+  return launch_sumfact<CT, 2, 2, QP, NB>(P, i, j, e0, e1, SP, out, out2, st);
 }
 
-}  // namespace hm
-
-using namespace hm;
-
-// The sum-factorized contraction when the problem carries tensor structure with 3 points per
-// direction and 1-D orders 1 or 2; returns HM_ERR_UNSUPPORTED otherwise (the caller then takes
-// another route).
-int hm_element_matrices_sumfact(const hm_problem* P, int i, int j, int64_t e0, int64_t e1, int precision,
-                                void* out, void* stream) {
-  if (P->nq1 != 3 || P->nq != 27 || !P->tab1_u || !P->tab1_m || !P->lex_u || !P->lex_m)
-    HM_FAIL(HM_ERR_UNSUPPORTED, "no tensor structure for the sum-factorized contraction");
-  if (P->nd_u > 32 || P->nd_m > 32) HM_FAIL(HM_ERR_UNSUPPORTED, "more than 32 dofs an element");
-  const int nt = i == HM_PARAMETER ? P->n1_m : P->n1_u;
-  const int nc = j == HM_PARAMETER ? P->n1_m : P->n1_u;
-  if (nt < 2 || nt > 3 || nc < 2 || nc > 3) HM_FAIL(HM_ERR_UNSUPPORTED, "1-D orders outside 1..2");
-  if (P->nd_u != P->n1_u * P->n1_u * P->n1_u || P->nd_m != P->n1_m * P->n1_m * P->n1_m)
-    HM_FAIL(HM_ERR_UNSUPPORTED, "the element dofs are not the tensor products of the 1-D nodes");
-  SumfactPlan SP = {};
-  // the nonzero (r, c) pairs of the block: the vv, vg, gv, gg parts of the density
-  const bool nl = P->density == HM_DENSITY_NONLINEAR;
+// The nonzero (r, c) pairs of block (i, j), r, c in {value, d/dxi_1, d/dxi_2, d/dxi_3}: the
+// vv, vg, gv, gg parts of the density; appended to pr/pc from np on, their k = 4 r + c bits in
+// *kmask.  Returns the new count.
+static int block_pairs(int i, int j, bool nl, int* pr, int* pc, int np, int* kmask) {
   bool vv = false, vg = false, gv = false, gg = false;
   switch (i * 3 + j) {
     case 0: vv = vg = gv = nl; break;
@@ -541,26 +555,57 @@ int hm_element_matrices_sumfact(const hm_problem* P, int i, int j, int64_t e0, i
     case 7: gv = true; break;
     default: break;
   }
-  int np = 0;
   for (int r = 0; r < 4; ++r)
     for (int c = 0; c < 4; ++c) {
       const bool on = (r == 0 && c == 0) ? vv : (r == 0) ? vg : (c == 0) ? gv : gg;
       if (on) {
-        SP.pr[np] = r;
-        SP.pc[np] = c;
-        SP.kmask |= 1 << (4 * r + c);
+        pr[np] = r;
+        pc[np] = c;
+        *kmask |= 1 << (4 * r + c);
         ++np;
       }
     }
-  SP.npair = np;
-  if (np == 0) {
-    const size_t n = (size_t)(e1 - e0) * (nt * nt * nt) * (nc * nc * nc);
-    return hm_fill_zero(out, n, precision == HM_FP64 ? 8 : 4, (hipStream_t)stream);
-  }
-  for (int p = 0; p < np; ++p) {
+  return np;
+}
+
+static void plan_masks(SumfactPlan& SP) {
+  for (int p = 0; p < SP.npair; ++p) {
     SP.tmask[((SP.pr[p] == 3) << 1) | (SP.pc[p] == 3)] |= 1 << p;
     if (SP.pr[p] >= 1 && SP.pc[p] >= 1 && SP.pr[p] != SP.pc[p]) SP.offdiag |= 1 << p;
   }
+}
+
+// the problem carries the tensor structure the sum-factorized contraction needs
+static int check_tensor(const hm_problem* P) {
+  if (P->nq1 != 3 || P->nq != 27 || !P->tab1_u || !P->tab1_m || !P->lex_u || !P->lex_m)
+    HM_FAIL(HM_ERR_UNSUPPORTED, "no tensor structure for the sum-factorized contraction");
+  if (P->nd_u > 32 || P->nd_m > 32) HM_FAIL(HM_ERR_UNSUPPORTED, "more than 32 dofs an element");
+  if (P->nd_u != P->n1_u * P->n1_u * P->n1_u || P->nd_m != P->n1_m * P->n1_m * P->n1_m)
+    HM_FAIL(HM_ERR_UNSUPPORTED, "the element dofs are not the tensor products of the 1-D nodes");
+  return HM_OK;
+}
+
+}  // namespace hm
+
+using namespace hm;
+
+// The sum-factorized contraction when the problem carries tensor structure with 3 points per
+// direction and 1-D orders 1 or 2; returns HM_ERR_UNSUPPORTED otherwise (the caller then takes
+// another route).
+int hm_element_matrices_sumfact(const hm_problem* P, int i, int j, int64_t e0, int64_t e1, int precision,
+                                void* out, void* stream) {
+  if (int rc = check_tensor(P)) return rc;
+  const int nt = i == HM_PARAMETER ? P->n1_m : P->n1_u;
+  const int nc = j == HM_PARAMETER ? P->n1_m : P->n1_u;
+  if (nt < 2 || nt > 3 || nc < 2 || nc > 3) HM_FAIL(HM_ERR_UNSUPPORTED, "1-D orders outside 1..2");
+  SumfactPlan SP = {};
+  const bool nl = P->density == HM_DENSITY_NONLINEAR;
+  SP.npair = block_pairs(i, j, nl, SP.pr, SP.pc, 0, &SP.kmask);
+  if (SP.npair == 0) {
+    const size_t n = (size_t)(e1 - e0) * (nt * nt * nt) * (nc * nc * nc);
+    return hm_fill_zero(out, n, precision == HM_FP64 ? 8 : 4, (hipStream_t)stream);
+  }
+  plan_masks(SP);
   SP.need_fields = !(i * 3 + j == 6 && !nl);  // the linear Jacobian reads m and the geometry only
   if (SP.need_fields && (P->n1_u < 2 || P->n1_u > 3))
     HM_FAIL(HM_ERR_UNSUPPORTED, "state fields of 1-D order outside 1..2 for the sum-factorized contraction");
@@ -572,4 +617,43 @@ int hm_element_matrices_sumfact(const hm_problem* P, int i, int j, int64_t e0, i
     case HM_FP16: return dispatch_orders<float, HM_FP16>(*P, i, j, e0, e1, nt, nc, SP, out, st);
     default: HM_FAIL(HM_ERR_UNSUPPORTED, "precision %d", precision);
   }
+}
+
+// Two blocks of one shape from one pass (FP64 or FP32); HM_ERR_UNSUPPORTED where the pair
+// does not qualify (different shapes, a block without pairs, more than 16 pairs together, or
+// no tensor structure): the caller then assembles the two blocks one by one.
+int hm_element_matrices_sumfact_pair(const hm_problem* P, int i1, int j1, int i2, int j2, int64_t e0, int64_t e1,
+                                     int precision, void* out1, void* out2, void* stream) {
+  if (int rc = check_tensor(P)) return rc;
+  const int nt = i1 == HM_PARAMETER ? P->n1_m : P->n1_u;
+  const int nc = j1 == HM_PARAMETER ? P->n1_m : P->n1_u;
+  if (nt != (i2 == HM_PARAMETER ? P->n1_m : P->n1_u) || nc != (j2 == HM_PARAMETER ? P->n1_m : P->n1_u))
+    HM_FAIL(HM_ERR_UNSUPPORTED, "blocks (%d, %d) and (%d, %d) differ in shape", i1, j1, i2, j2);
+  if (nt < 2 || nt > 3 || nc < 2 || nc > 3) HM_FAIL(HM_ERR_UNSUPPORTED, "1-D orders outside 1..2");
+  if (precision != HM_FP64 && precision != HM_FP32)
+    HM_FAIL(HM_ERR_UNSUPPORTED, "two blocks a pass in FP64 or FP32 only");
+  SumfactPlan SP = {};
+  const bool nl = P->density == HM_DENSITY_NONLINEAR;
+  int pr2[SF_MAXPAIR], pc2[SF_MAXPAIR];
+  const int n1 = block_pairs(i1, j1, nl, SP.pr, SP.pc, 0, &SP.kmask);
+  const int n2 = block_pairs(i2, j2, nl, pr2, pc2, 0, &SP.kmask2);
+  if (n1 == 0 || n2 == 0 || n1 + n2 > SF_MAXPAIR)
+    HM_FAIL(HM_ERR_UNSUPPORTED, "blocks (%d, %d) and (%d, %d) do not share a pass", i1, j1, i2, j2);
+  for (int p = 0; p < n2; ++p) {
+    SP.pr[n1 + p] = pr2[p];
+    SP.pc[n1 + p] = pc2[p];
+  }
+  SP.npair = n1 + n2;
+  SP.bmask[0] = (1 << n1) - 1;
+  SP.bmask[1] = ((1 << n2) - 1) << n1;
+  SP.s2 = i2;
+  SP.j2 = j2;
+  plan_masks(SP);
+  SP.need_fields = !(i1 * 3 + j1 == 6 && !nl) || !(i2 * 3 + j2 == 6 && !nl);
+  if (SP.need_fields && (P->n1_u < 2 || P->n1_u > 3))
+    HM_FAIL(HM_ERR_UNSUPPORTED, "state fields of 1-D order outside 1..2 for the sum-factorized contraction");
+  hipStream_t st = (hipStream_t)stream;
+  if (precision == HM_FP64)
+    return dispatch_orders<double, HM_FP64, 2>(*P, i1, j1, e0, e1, nt, nc, SP, out1, st, out2);
+  return dispatch_orders<float, HM_FP32, 2>(*P, i1, j1, e0, e1, nt, nc, SP, out1, st, out2);
 }

@@ -132,10 +132,48 @@ class QuadratureKernel:
         self._keep = (self._keep, K, Rj, Ri)
         return out
 
+    def element_matrices_pair(self, ij1, ij2, local_arrays, precision="fp64", elements=None,
+                              outs=(None, None), stream=None):
+        """Blocks ``ij1`` and ``ij2`` of one shape from one pass over the elements: the
+        geometry, the fields and both blocks' pointwise values at the points once, then the two
+        contractions.  Each result has the bits ``element_matrices`` gives it.  ``None``, with
+        nothing written, where the two do not share a pass (another shape, a block that is
+        zero for the density, the BF16/FP16 forms, more pointwise terms than the kernel holds)."""
+        if precision not in ("fp64", "fp32"):
+            return None
+        (i1, j1), (i2, j2) = ij1, ij2
+        if (self.nd[i1], self.nd[j1]) != (self.nd[i2], self.nd[j2]):
+            return None
+        P, keep = self.problem(local_arrays)
+        a, b = (0, self.group.ne) if elements is None else elements
+        code = _lib.PRECISIONS[precision]
+        dtype = torch.float64 if code == _lib.FP64 else torch.float32
+        o1, o2 = (o if o is not None else _result_array((b - a, self.nd[i1], self.nd[j1]), dtype,
+                                                        keep[0].device) for o in outs)
+        rc = _lib.lib().hm_element_matrices_pair(
+            _lib.ctypes.byref(P), int(i1), int(j1), int(i2), int(j2), int(a), int(b), code,
+            _lib.ptr(o1), _lib.ptr(o2), _lib.stream_handle(stream))
+        if rc == _lib.HM_ERR_UNSUPPORTED:
+            return None
+        _lib.check(rc, "element matrices (%d, %d) and (%d, %d)" % (i1, j1, i2, j2))
+        return o1, o2
+
     def element_matrices_many(self, pairs, local_arrays, precision="fp64"):
-        """Several blocks from one call (``element_matrices_many``, ``kernel.py:2048-2068``)."""
-        return {ij: self.element_matrices(ij[0], ij[1], local_arrays, precision=precision)
-                for ij in pairs}
+        """Several blocks from one call (``element_matrices_many``, ``kernel.py:2048-2068``):
+        blocks of one shape two at a time from one pass where the kernel allows, the others
+        one by one."""
+        out, todo = {}, list(dict.fromkeys(tuple(ij) for ij in pairs))
+        while todo:
+            ij = todo.pop(0)
+            for k, other in enumerate(todo):
+                got = self.element_matrices_pair(ij, other, local_arrays, precision=precision)
+                if got is not None:
+                    out[ij], out[other] = [got[0]], [got[1]]
+                    del todo[k]
+                    break
+            else:
+                out[ij] = self.element_matrices(ij[0], ij[1], local_arrays, precision=precision)
+        return out
 
     def element_matrix_chunks(self, i, j, local_arrays, chunk=None, precision="fp64"):
         """A generator of ``(group, start, stop, array)`` chunks (``element_matrix_chunks``,

@@ -103,12 +103,15 @@ class BlockAssembly:
         return self.pattern.data([self.E], plan=self.dplan, out=self.data, stream=stream,
                                  row_defect=self.row_defect)
 
-    def assemble(self, local_arrays, stream=None):
-        self.element_matrices(local_arrays, stream)
-        self.scatter(stream)
+    def finish(self):
         A = finish_block(self.plan, self.data)
         A.row_defect = self.row_defect
         return A
+
+    def assemble(self, local_arrays, stream=None):
+        self.element_matrices(local_arrays, stream)
+        self.scatter(stream)
+        return self.finish()
 
 
 class PDEVariationalProblem:
@@ -157,6 +160,33 @@ class PDEVariationalProblem:
         require_finite_arrays([blk.E], "block (%d, %d)" % (i, j))
         return A
 
+    def blocks(self, pairs, x, precision="fp64"):
+        """Blocks ``pairs`` at ``x``, each as ``block`` gives it, with two blocks of one shape
+        from one pass over the elements wherever the kernel allows (the reference takes every
+        linearization point's blocks from one differentiation pass, ``element_matrices_many``,
+        ``kernel.py:2048-2068``)."""
+        loc = self._locals(x)
+        todo = list(dict.fromkeys(tuple(ij) for ij in pairs))
+        out = {}
+        while todo:
+            ij = todo.pop(0)
+            blk = self.block_assembly(ij[0], ij[1], precision)
+            done = [ij]
+            for k, other in enumerate(todo):
+                ob = self.block_assembly(other[0], other[1], precision)
+                if self.kernel.element_matrices_pair(ij, other, loc, precision=precision,
+                                                     outs=(blk.E, ob.E)) is not None:
+                    done.append(todo.pop(k))
+                    break
+            else:
+                blk.element_matrices(loc)
+            for d in done:
+                b = self.block_assembly(d[0], d[1], precision)
+                require_finite_arrays([b.E], "block (%d, %d)" % d)
+                b.scatter()
+                out[d] = b.finish()
+        return out
+
     def jacobian(self, x, precision="fp64"):
         """The forward Jacobian ``A`` (``_jacobian``, ``PDEVariationalProblem.py:542-659``):
         the parameter (and, for a nonlinear residual, the state) must be finite
@@ -169,15 +199,15 @@ class PDEVariationalProblem:
     def set_linearization_point(self, x, gauss_newton_approx=False, precision="fp64"):
         """``A`` and the Hessian blocks C, W_uu (nonlinear state), W_um, W_mm
         (``setLinearizationPoint``, ``PDEVariationalProblem.py:1372-1482``)."""
-        need = [(ADJOINT, PARAMETER)]
+        require_finite(x[PARAMETER], "the parameter")
+        if not self.is_fwd_linear:
+            require_finite(x[STATE], "the state")
+        need = [(ADJOINT, STATE), (ADJOINT, PARAMETER)]
         if not gauss_newton_approx:
             if not self.is_fwd_linear:
                 need.append((STATE, STATE))
             need += [(STATE, PARAMETER), (PARAMETER, PARAMETER)]
-        out = {(ADJOINT, STATE): self.jacobian(x, precision)}
-        for ij in need:
-            out[ij] = self.block(ij[0], ij[1], x, precision)
-        return out
+        return self.blocks(need, x, precision)
 
     def apply_ij_at(self, i, j, x, dir, out=None):
         """Block ``(i, j)`` at ``x`` applied to ``dir`` without assembling it

@@ -215,6 +215,76 @@ def test_non_finite_raises(gpu):
         prob.jacobian(x)
 
 
+@pytest.mark.parametrize("perturb", [0.0, 0.2])
+@pytest.mark.parametrize("nonlinear", [False, True])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_pair_pass_equals_single_blocks(gpu, precision, nonlinear, perturb):
+    """Two blocks of one shape from one pass over the elements have the bits of the two
+    single-block launches, for every pair of blocks that shares a pass; pairs that do not
+    (other shapes, a block that is zero for the density, BF16) are refused and write nothing."""
+    W, P = pair(3, perturb=perturb, nonlinear=nonlinear)
+    loc = W.locals()
+    K = W.kernel
+    single = {(i, j): K.element_matrices(i, j, loc, precision=precision)[0].clone()
+              for i in range(3) for j in range(3)}
+    shared = set()
+    for a in range(9):
+        for b in range(9):
+            ij1, ij2 = divmod(a, 3), divmod(b, 3)
+            if a == b:
+                continue
+            got = K.element_matrices_pair(ij1, ij2, loc, precision=precision)
+            if got is None:
+                continue
+            shared.add((ij1, ij2))
+            assert (K.nd[ij1[0]], K.nd[ij1[1]]) == (K.nd[ij2[0]], K.nd[ij2[1]])
+            assert bool(single[ij1].abs().max() > 0) and bool(single[ij2].abs().max() > 0)
+            assert torch.equal(got[0], single[ij1]), (ij1, ij2)
+            assert torch.equal(got[1], single[ij2]), (ij1, ij2)
+    # C and W_um (and their transposed counterparts) share a pass at either density
+    assert ((ADJOINT, PARAMETER), (STATE, PARAMETER)) in shared
+    assert ((PARAMETER, ADJOINT), (PARAMETER, STATE)) in shared
+    # a refused pair leaves the outputs as they were
+    o1 = torch.full_like(single[(ADJOINT, STATE)], 7.0)
+    o2 = torch.full_like(single[(ADJOINT, STATE)], 7.0)
+    assert K.element_matrices_pair((ADJOINT, STATE), (ADJOINT, PARAMETER), loc,
+                                   precision=precision, outs=(o1, o2)) is None
+    assert K.element_matrices_pair((ADJOINT, STATE), (STATE, STATE), loc, precision="bf16",
+                                   outs=(o1, o2)) is None
+    assert bool((o1 == 7.0).all()) and bool((o2 == 7.0).all())
+    many = K.element_matrices_many([(ADJOINT, PARAMETER), (STATE, PARAMETER),
+                                    (ADJOINT, STATE)], loc, precision=precision)
+    for ij, E in many.items():
+        assert torch.equal(E[0], single[ij]), ij
+
+
+@pytest.mark.parametrize("nonlinear", [False, True])
+def test_linearization_point_blocks(gpu, nonlinear):
+    """The blocks of a linearization point, taken two at a time from one element pass where
+    they share one, are the blocks ``block`` assembles one by one, bit for bit, and match the
+    oracle's assembled blocks."""
+    from hippymfem_amd.problem import PDEVariationalProblem
+
+    W, P = pair(4, perturb=0.2, nonlinear=nonlinear)
+    prob = PDEVariationalProblem(W.spaces, W.density, W.batches, W.bc,
+                                 is_fwd_linear=not nonlinear)
+    x = [W.u.copy(), W.m.copy(), W.p.copy()]
+    got = prob.set_linearization_point(x)
+    want = [(ADJOINT, STATE), (ADJOINT, PARAMETER), (STATE, PARAMETER), (PARAMETER, PARAMETER)]
+    if nonlinear:
+        want.append((STATE, STATE))
+    assert sorted(got) == sorted(want)
+    for ij in want:
+        ref = prob.block(ij[0], ij[1], x)
+        ip, ix, d = got[ij].host_csr()
+        ip1, ix1, d1 = ref.host_csr()
+        assert np.array_equal(ip, ip1) and np.array_equal(ix, ix1)
+        assert np.array_equal(d, d1), ij
+        ipo, ixo, do = P.assembled(*ij)
+        assert np.array_equal(ip, ipo) and np.array_equal(ix, ixo)
+        assert rel(d, do) <= TOL, (ij, rel(d, do))
+
+
 def test_batch_that_does_not_fit_raises_memory_error(gpu):
     """An element batch whose result does not fit raises MemoryError (the reference's
     message when no chunk fits, kernel.py:1096-1101): a result of 10^8 Q2 element matrices
