@@ -26,7 +26,7 @@
 
 namespace hm {
 
-constexpr int SF_MAXPAIR = 16;
+constexpr int SF_MAXPAIR = 32;  // 16 (r, c) pairs a block, two blocks a pass
 
 struct SumfactPlan {
   int npair;
@@ -363,7 +363,8 @@ __attribute__((amdgpu_waves_per_eu(sf_waves<CT, NT, NC>(), sf_waves<CT, NT, NC>(
     // 3. the pairs this element needs.  The low-precision forms also measure every pair
     // against the element's largest pointwise weight, their scale, and leave out the pairs
     // below SF_DROP of it.
-    uint32_t live = ((1u << npair) - 1u) & (aligned ? ~(uint32_t)SP.offdiag : ~0u);
+    const uint32_t every = npair >= 32 ? ~0u : (1u << npair) - 1u;
+    uint32_t live = every & (aligned ? ~(uint32_t)SP.offdiag : ~0u);
     double top = 1.0;
     if constexpr (LOWP) {
       double m0 = 0.0, m1 = 0.0;
@@ -379,7 +380,7 @@ __attribute__((amdgpu_waves_per_eu(sf_waves<CT, NT, NC>(), sf_waves<CT, NT, NC>(
       const uint64_t bal0 = __ballot(act && r < npair && m0 > SF_DROP * top);
       const uint64_t bal1 = __ballot(act && LPE + r < npair && m1 > SF_DROP * top);
       const uint64_t both = ((bal0 >> sh) & lpe_mask) | (((bal1 >> sh) & lpe_mask) << LPE);
-      live = (uint32_t)both & ((1u << npair) - 1u);
+      live = (uint32_t)both & every;
     }
     // 4. lane = contraction task (a1, b1, x2).  The low-precision forms scale the element's
     // pairs by its largest pointwise weight (operands within [-1, 1]) and the result back.
@@ -570,8 +571,8 @@ static int block_pairs(int i, int j, bool nl, int* pr, int* pc, int np, int* kma
 
 static void plan_masks(SumfactPlan& SP) {
   for (int p = 0; p < SP.npair; ++p) {
-    SP.tmask[((SP.pr[p] == 3) << 1) | (SP.pc[p] == 3)] |= 1 << p;
-    if (SP.pr[p] >= 1 && SP.pc[p] >= 1 && SP.pr[p] != SP.pc[p]) SP.offdiag |= 1 << p;
+    SP.tmask[((SP.pr[p] == 3) << 1) | (SP.pc[p] == 3)] |= (int)(1u << p);
+    if (SP.pr[p] >= 1 && SP.pc[p] >= 1 && SP.pr[p] != SP.pc[p]) SP.offdiag |= (int)(1u << p);
   }
 }
 
@@ -620,8 +621,8 @@ int hm_element_matrices_sumfact(const hm_problem* P, int i, int j, int64_t e0, i
 }
 
 // Two blocks of one shape from one pass (FP64 or FP32); HM_ERR_UNSUPPORTED where the pair
-// does not qualify (different shapes, a block without pairs, more than 16 pairs together, or
-// no tensor structure): the caller then assembles the two blocks one by one.
+// does not qualify (different shapes, a block without pairs, or no tensor structure): the
+// caller then assembles the two blocks one by one.
 int hm_element_matrices_sumfact_pair(const hm_problem* P, int i1, int j1, int i2, int j2, int64_t e0, int64_t e1,
                                      int precision, void* out1, void* out2, void* stream) {
   if (int rc = check_tensor(P)) return rc;
@@ -644,8 +645,8 @@ int hm_element_matrices_sumfact_pair(const hm_problem* P, int i1, int j1, int i2
     SP.pc[n1 + p] = pc2[p];
   }
   SP.npair = n1 + n2;
-  SP.bmask[0] = (1 << n1) - 1;
-  SP.bmask[1] = ((1 << n2) - 1) << n1;
+  SP.bmask[0] = (int)((1u << n1) - 1u);
+  SP.bmask[1] = (int)(((1u << n2) - 1u) << n1);
   SP.s2 = i2;
   SP.j2 = j2;
   plan_masks(SP);

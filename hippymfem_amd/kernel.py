@@ -138,7 +138,7 @@ class QuadratureKernel:
         geometry, the fields and both blocks' pointwise values at the points once, then the two
         contractions.  Each result has the bits ``element_matrices`` gives it.  ``None``, with
         nothing written, where the two do not share a pass (another shape, a block that is
-        zero for the density, the BF16/FP16 forms, more pointwise terms than the kernel holds)."""
+        zero for the density, the BF16/FP16 forms)."""
         if precision not in ("fp64", "fp32"):
             return None
         (i1, j1), (i2, j2) = ij1, ij2

@@ -95,8 +95,8 @@ int hm_element_matrices(const hm_problem* P, int i, int j, int64_t e0, int64_t e
  * elements: the geometry, the fields and the pointwise pairs of both blocks at the points once,
  * then the two contractions; out1 and out2 as hm_element_matrices writes them, to the same bits.
  * HM_FP64 or HM_FP32.  Returns HM_ERR_UNSUPPORTED, having written nothing, where the blocks do
- * not share a pass (different shapes, a block that is zero for the density, more than 16
- * pointwise pairs together, no tensor structure): the caller then makes two single calls.
+ * not share a pass (different shapes, a block that is zero for the density, no tensor
+ * structure): the caller then makes two single calls.
  * Replaces GroupKernel.element_matrices_many (hippymfem/fem/kernel.py:2048-2068) for two blocks. */
 int hm_element_matrices_pair(const hm_problem* P, int i1, int j1, int i2, int j2, int64_t e0,
                              int64_t e1, int precision, void* out1, void* out2, void* stream);

@@ -241,9 +241,12 @@ def test_pair_pass_equals_single_blocks(gpu, precision, nonlinear, perturb):
             assert bool(single[ij1].abs().max() > 0) and bool(single[ij2].abs().max() > 0)
             assert torch.equal(got[0], single[ij1]), (ij1, ij2)
             assert torch.equal(got[1], single[ij2]), (ij1, ij2)
-    # C and W_um (and their transposed counterparts) share a pass at either density
+    # C and W_um (and their transposed counterparts) share a pass at either density, and so
+    # do A and W_uu of the nonlinear density (19 pointwise pairs together)
     assert ((ADJOINT, PARAMETER), (STATE, PARAMETER)) in shared
     assert ((PARAMETER, ADJOINT), (PARAMETER, STATE)) in shared
+    if nonlinear:
+        assert ((ADJOINT, STATE), (STATE, STATE)) in shared
     # a refused pair leaves the outputs as they were
     o1 = torch.full_like(single[(ADJOINT, STATE)], 7.0)
     o2 = torch.full_like(single[(ADJOINT, STATE)], 7.0)
