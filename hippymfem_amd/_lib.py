@@ -38,7 +38,7 @@ class HmProblem(ctypes.Structure):
                 ("f", DBL), ("verts", P), ("elem_verts", P), ("edofs_u", P), ("edofs_m", P),
                 ("tab_u", P), ("tab_m", P), ("tab_x", P), ("wq", P), ("u", P), ("m", P),
                 ("p", P), ("nq1", I32), ("n1_u", I32), ("n1_m", I32), ("reserved", I32),
-                ("tab1_u", P), ("tab1_m", P), ("lex_u", P), ("lex_m", P)]
+                ("tab1_u", P), ("tab1_m", P), ("lex_u", P), ("lex_m", P), ("nonfinite", P)]
 
 
 class HmScatterPlan(ctypes.Structure):

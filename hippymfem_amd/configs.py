@@ -7,7 +7,13 @@
 * ``nonlinear``: ``exp(m) (1 + u^2) grad u . grad p - f p`` at a smooth state (8 Fourier
   modes, amplitude 0.5): the residual and every block in FP64 and FP32, times and errors.
 
-``python -m hippymfem_amd.configs precision --n 128`` / ``nonlinear --n 96``.
+* ``linearization``: the blocks of a linearization point (A, C, W_um, W_mm, and W_uu at the
+  nonlinear density) assembled block by block (``block``), and with the blocks of one shape
+  two at a time from one element pass (``blocks``); FP64, both with the finiteness check of
+  every element matrix; the two times and whether the values are the same bits.
+
+``python -m hippymfem_amd.configs precision --n 128`` / ``nonlinear --n 96`` /
+``linearization --n 64 [--nonlinear]``.
 """
 
 import argparse
@@ -96,15 +102,41 @@ def nonlinear(n=96, steps=3):
     return _blocks(prob, loc, blocks, ("fp32",), steps, n)
 
 
+def linearization(n=64, nonlinear=False, steps=3):
+    W = Workload(n, perturb=0.2, nonlinear=nonlinear, state="fourier" if nonlinear else "uniform")
+    prob = PDEVariationalProblem(W.spaces, W.density, W.batches, W.bc,
+                                 is_fwd_linear=not nonlinear)
+    loc = [torch.as_tensor(v).cuda() for v in W.locals()]
+    x = [loc[0], loc[1], loc[2]]
+    need = [(ADJOINT, STATE), (ADJOINT, PARAMETER), (STATE, PARAMETER), (PARAMETER, PARAMETER)]
+    if nonlinear:
+        need.insert(2, (STATE, STATE))
+    blks = [prob.block_assembly(*ij) for ij in need]
+    t_sep = _time(lambda: [prob.block(ij[0], ij[1], x) for ij in need], steps)
+    sep = [b.data.clone() for b in blks]
+    t_shr = _time(lambda: prob.blocks(need, x), steps)
+    same = all(torch.equal(b.data, d) for b, d in zip(blks, sep))
+    line = {"n": n, "density": "nonlinear" if nonlinear else "linear",
+            "blocks": [NAMES[ij] for ij in need],
+            "nnz": int(sum(b.pattern.nnz for b in blks)),
+            "separate_ms": t_sep, "shared_ms": t_shr, "speedup": t_sep / t_shr,
+            "same_bits": bool(same)}
+    print(json.dumps(line), flush=True)
+    return line
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__)
-    ap.add_argument("which", choices=["precision", "nonlinear"])
+    ap.add_argument("which", choices=["precision", "nonlinear", "linearization"])
     ap.add_argument("--n", type=int, default=None)
+    ap.add_argument("--nonlinear", action="store_true")
     ap.add_argument("--steps", type=int, default=3)
     a = ap.parse_args(argv)
     t0 = time.perf_counter()
     if a.which == "precision":
         precision_form(a.n or 128, steps=a.steps)
+    elif a.which == "linearization":
+        linearization(a.n or 64, nonlinear=a.nonlinear, steps=a.steps)
     else:
         nonlinear(a.n or 96, steps=a.steps)
     print(json.dumps({"wall_s": time.perf_counter() - t0}), flush=True)

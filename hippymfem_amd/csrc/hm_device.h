@@ -31,3 +31,6 @@ int hm_element_matrices_sumfact_pair(const hm_problem* P, int i1, int j1, int i2
                                      int64_t e1, int precision, void* out1, void* out2, void* stream);
 // Fills n words of 4 or 8 bytes with zero on the stream (a kernel: safe under capture).
 int hm_fill_zero(void* p, size_t n, int word_bytes, hipStream_t st);
+// Sets *flag to 1 on the stream where one of the n words (fp64 for 8 bytes, fp32 for 4) is
+// not finite; leaves it otherwise.  Nothing where flag is NULL.
+int hm_flag_nonfinite(const void* p, size_t n, int word_bytes, int32_t* flag, hipStream_t st);

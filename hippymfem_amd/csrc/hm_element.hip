@@ -284,8 +284,13 @@ extern "C" int hm_element_matrices(const hm_problem* P, int i, int j, int64_t e0
     const int rc = hm_element_matrices_sumfact(P, i, j, e0, e1, precision, out, stream);
     if (rc != HM_ERR_UNSUPPORTED) return rc;
   }
-  if (nd_ti <= 32 && nd_cj <= 32 && (!valu || precision == HM_FP32))
-    return hm_element_matrices_mfma(P, i, j, e0, e1, precision, out, stream);
+  // the other routes: a pass over the written matrices sets the non-finite flag
+  const size_t nent = (size_t)(e1 - e0) * nd_ti * nd_cj;
+  if (nd_ti <= 32 && nd_cj <= 32 && (!valu || precision == HM_FP32)) {
+    const int rc = hm_element_matrices_mfma(P, i, j, e0, e1, precision, out, stream);
+    if (rc) return rc;
+    return hm_flag_nonfinite(out, nent, precision == HM_FP64 ? 8 : 4, P->nonfinite, (hipStream_t)stream);
+  }
   HM_REQUIRE(precision == HM_FP64, "the FP32 contraction needs at most 32 dofs per element");
   const int nq = P->nq;
   const int nd_t = i == HM_PARAMETER ? P->nd_m : P->nd_u;
@@ -299,7 +304,8 @@ extern "C" int hm_element_matrices(const hm_problem* P, int i, int j, int64_t e0
   hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(k_element_matrices<double>, dim3(grid_for(e1 - e0)), dim3(256), bytes, s, *P, i,
                      j, e0, e1, parts, (double*)out);
-  return hip_check("hm_element_matrices");
+  if (int rc = hip_check("hm_element_matrices")) return rc;
+  return hm_flag_nonfinite(out, nent, 8, P->nonfinite, s);
 }
 
 extern "C" int hm_element_matrices_pair(const hm_problem* P, int i1, int j1, int i2, int j2, int64_t e0,

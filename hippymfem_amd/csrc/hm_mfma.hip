@@ -251,7 +251,26 @@ __global__ void k_fill_zero8(uint64_t* p, size_t n) {
 __global__ void k_fill_zero4(uint32_t* p, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0;
 }
+// *flag = 1 where a value is not finite (an exponent of all ones: infinity or NaN)
+template <typename T>
+__global__ void k_flag_nonfinite(const T* __restrict__ p, size_t n, int32_t* flag) {
+  bool bad = false;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    bad |= !__builtin_isfinite(p[i]);
+  if (bad) atomicOr(flag, 1);
+}
 }  // namespace hm
+
+int hm_flag_nonfinite(const void* p, size_t n, int word_bytes, int32_t* flag, hipStream_t st) {
+  if (n == 0 || flag == nullptr) return HM_OK;
+  const size_t nb = (n + 255) / 256;
+  const int grid = (int)(nb < 8192 ? nb : 8192);
+  if (word_bytes == 8)
+    hipLaunchKernelGGL(k_flag_nonfinite<double>, dim3(grid), dim3(256), 0, st, (const double*)p, n, flag);
+  else
+    hipLaunchKernelGGL(k_flag_nonfinite<float>, dim3(grid), dim3(256), 0, st, (const float*)p, n, flag);
+  return hip_check("flag non-finite");
+}
 
 int hm_fill_zero(void* p, size_t n, int word_bytes, hipStream_t st) {
   if (n == 0) return HM_OK;

@@ -273,6 +273,7 @@ __attribute__((amdgpu_waves_per_eu(sf_waves<CT, NT, NC>(), sf_waves<CT, NT, NC>(
   CT acc[N2][NT][NC];
   int64_t prev_el = -1;
   double prev_scale = 1.0;
+  bool bad = false;  // a value stored was not finite
   auto store_acc = [&](CT* dst, int64_t e_, double scale) {
     CT* Eo = dst + (size_t)e_ * nd_t * nd_c;
 #pragma unroll
@@ -281,8 +282,11 @@ __attribute__((amdgpu_waves_per_eu(sf_waves<CT, NT, NC>(), sf_waves<CT, NT, NC>(
       for (int i = 0; i < N2; ++i) {
         CT* Er = Eo + lt[a1 + NT * (a2_of(i) + NT * a3)] * nd_c;
 #pragma unroll
-        for (int b3 = 0; b3 < NC; ++b3)
-          Er[lc[b1 + NC * (b2_of(i) + NC * b3)]] = LOWP ? (CT)(acc[i][a3][b3] * scale) : acc[i][a3][b3];
+        for (int b3 = 0; b3 < NC; ++b3) {
+          const CT v = LOWP ? (CT)(acc[i][a3][b3] * scale) : acc[i][a3][b3];
+          Er[lc[b1 + NC * (b2_of(i) + NC * b3)]] = v;
+          bad |= !__builtin_isfinite(v);
+        }
       }
   };
   auto store_prev = [&]() {
@@ -494,6 +498,7 @@ __attribute__((amdgpu_waves_per_eu(sf_waves<CT, NT, NC>(), sf_waves<CT, NT, NC>(
     wave_sync();
   }
   store_prev();
+  if (bad && P.nonfinite) atomicOr(P.nonfinite, 1);
 }
 
 template <typename CT, int NT, int NC, int QP, int NB>

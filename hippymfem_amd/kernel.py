@@ -60,8 +60,10 @@ class QuadratureKernel:
         self._keep = None
 
     # ------------------------------------------------------------- plumbing
-    def problem(self, local_arrays):
-        """The ``hm_problem`` of the current fields; keeps the device arrays alive."""
+    def problem(self, local_arrays, nonfinite=None):
+        """The ``hm_problem`` of the current fields; keeps the device arrays alive.
+        ``nonfinite``: an int32 device flag the element-matrix kernels set to 1 when they
+        write a value that is not finite."""
         dev = _lib.require_device()
         g = self.group.device()
         tu = self.tables[STATE].device()
@@ -77,23 +79,26 @@ class QuadratureKernel:
             nq1=self.group.nq1 if self.group.nq1 ** 3 == self.group.nq else 0,
             n1_u=self.tables[STATE].n1, n1_m=self.tables[PARAMETER].n1,
             tab1_u=_lib.ptr(tu["tab1"]), tab1_m=_lib.ptr(tm["tab1"]),
-            lex_u=_lib.ptr(tu["lex"]), lex_m=_lib.ptr(tm["lex"]))
+            lex_u=_lib.ptr(tu["lex"]), lex_m=_lib.ptr(tm["lex"]),
+            nonfinite=None if nonfinite is None else _lib.ptr(nonfinite))
         self._keep = (u, m, p, g, tu, tm)
         return P, (u, m, p)
 
     # ------------------------------------------------------------ element arrays
     def element_matrices(self, i, j, local_arrays, params=(), consistent=None,
-                         precision="fp64", elements=None, out=None, stream=None):
+                         precision="fp64", elements=None, out=None, stream=None, nonfinite=None):
         """Block ``(i, j)`` element matrices ``[(ne, nd_i, nd_j)]`` (``element_matrices``,
         ``kernel.py:1964-2006``).  ``precision``: ``"fp64"`` (default), or ``"fp32"``,
         ``"bf16"``, ``"fp16"`` for the precision form (fp32 results).  ``consistent``
         (``"symmetric"`` or ``"two-sided"``) with a precision form: the consistent block of
-        ``_block_fn`` (``kernel.py:1573-1580``), fp64 matrices exact on the constants."""
+        ``_block_fn`` (``kernel.py:1573-1580``), fp64 matrices exact on the constants.
+        ``nonfinite``: an int32 device flag set to 1 when an entry written is not finite (the
+        caller clears it)."""
         if consistent and precision != "fp64":
             return [self.consistent_block(i, j, local_arrays, consistent == "symmetric",
                                           precision=precision, elements=elements, out=out,
                                           stream=stream)]
-        P, keep = self.problem(local_arrays)
+        P, keep = self.problem(local_arrays, nonfinite)
         a, b = (0, self.group.ne) if elements is None else elements
         code = _lib.PRECISIONS[precision]
         dtype = torch.float64 if code == _lib.FP64 else torch.float32
@@ -133,7 +138,7 @@ class QuadratureKernel:
         return out
 
     def element_matrices_pair(self, ij1, ij2, local_arrays, precision="fp64", elements=None,
-                              outs=(None, None), stream=None):
+                              outs=(None, None), stream=None, nonfinite=None):
         """Blocks ``ij1`` and ``ij2`` of one shape from one pass over the elements: the
         geometry, the fields and both blocks' pointwise values at the points once, then the two
         contractions.  Each result has the bits ``element_matrices`` gives it.  ``None``, with
@@ -144,7 +149,7 @@ class QuadratureKernel:
         (i1, j1), (i2, j2) = ij1, ij2
         if (self.nd[i1], self.nd[j1]) != (self.nd[i2], self.nd[j2]):
             return None
-        P, keep = self.problem(local_arrays)
+        P, keep = self.problem(local_arrays, nonfinite)
         a, b = (0, self.group.ne) if elements is None else elements
         code = _lib.PRECISIONS[precision]
         dtype = torch.float64 if code == _lib.FP64 else torch.float32

@@ -6,6 +6,8 @@ element matrices and the CSR values, so a repeated assembly (a Newton step, a se
 request, a benchmark step) launches kernels into resident memory and allocates nothing.
 """
 
+import contextlib
+
 import numpy as np
 import torch
 
@@ -94,10 +96,21 @@ class BlockAssembly:
         self.row_defect = None
         if row_sums and Vi is Vj and value_dtype != torch.float64:
             self.row_defect = torch.empty(self.pattern.nrow, dtype=torch.float64, device=dev)
+        # set by the element kernel when it writes a value that is not finite
+        self.nonfinite = torch.zeros(1, dtype=torch.int32, device=dev)
+
+    def clear_flag(self, stream=None):
+        with torch.cuda.stream(stream) if stream is not None else contextlib.nullcontext():
+            self.nonfinite.zero_()
 
     def element_matrices(self, local_arrays, stream=None):
+        self.clear_flag(stream)
         return self.kernel.element_matrices(self.i, self.j, local_arrays, precision=self.precision,
-                                            out=self.E, stream=stream)[0]
+                                            out=self.E, stream=stream, nonfinite=self.nonfinite)[0]
+
+    def finite(self):
+        """Whether the last element pass wrote finite values only (one int32 read back)."""
+        return not bool(self.nonfinite.item())
 
     def scatter(self, stream=None):
         return self.pattern.data([self.E], plan=self.dplan, out=self.data, stream=stream,
@@ -157,7 +170,8 @@ class PDEVariationalProblem:
         not finite (the check of ``_domain_block``, ``:495-525``)."""
         blk = self.block_assembly(i, j, precision)
         A = blk.assemble(self._locals(x))
-        require_finite_arrays([blk.E], "block (%d, %d)" % (i, j))
+        if not blk.finite():
+            require_finite_arrays([blk.E], "block (%d, %d)" % (i, j))
         return A
 
     def blocks(self, pairs, x, precision="fp64"):
@@ -171,18 +185,24 @@ class PDEVariationalProblem:
         while todo:
             ij = todo.pop(0)
             blk = self.block_assembly(ij[0], ij[1], precision)
+            blk.clear_flag()
             done = [ij]
             for k, other in enumerate(todo):
                 ob = self.block_assembly(other[0], other[1], precision)
                 if self.kernel.element_matrices_pair(ij, other, loc, precision=precision,
-                                                     outs=(blk.E, ob.E)) is not None:
+                                                     outs=(blk.E, ob.E),
+                                                     nonfinite=blk.nonfinite) is not None:
                     done.append(todo.pop(k))
                     break
             else:
                 blk.element_matrices(loc)
+            # one flag for the pass (the first block's): a set flag is traced to its block
+            if not blk.finite():
+                for d in done:
+                    require_finite_arrays([self.block_assembly(d[0], d[1], precision).E],
+                                          "block (%d, %d)" % d)
             for d in done:
                 b = self.block_assembly(d[0], d[1], precision)
-                require_finite_arrays([b.E], "block (%d, %d)" % d)
                 b.scatter()
                 out[d] = b.finish()
         return out

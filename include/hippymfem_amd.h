@@ -83,6 +83,12 @@ typedef struct hm_problem {
   const double* tab1_m;
   const int32_t* lex_u;
   const int32_t* lex_m;
+  /* NULL, or one int32 the element-matrix entries set to 1 when one of the values they write
+   * is not finite (the caller clears it; it is never cleared here).  The reference's callers
+   * check every element matrix so (require_finite_arrays,
+   * hippymfem/modeling/PDEVariationalProblem.py:1844-1866); here the kernels check the values
+   * as they store them and no second pass reads the matrices. */
+  int32_t* nonfinite;
 } hm_problem;
 
 /* Element matrices of block (i, j) for elements [e0, e1): out (e1-e0, nd_i, nd_j), fp64

@@ -288,6 +288,47 @@ def test_linearization_point_blocks(gpu, nonlinear):
         assert rel(d, do) <= TOL, (ij, rel(d, do))
 
 
+@pytest.mark.parametrize("route", ["sumfact", "mfma", "valu"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16"])
+def test_nonfinite_flag(gpu, route, precision, monkeypatch):
+    """The element-matrix kernels set the caller's flag exactly when a value they write is
+    not finite (checked against torch.isfinite on the same output), on every contraction
+    route; the two-block pass too; and a linearization point raises RuntimeError."""
+    from hippymfem_amd.problem import PDEVariationalProblem
+
+    if route == "valu" and precision != "fp64" or route != "sumfact" and precision == "bf16":
+        pytest.skip("the route has no such precision")
+    monkeypatch.setenv("HM_CONTRACTION", route)
+    W = Workload(3)
+    K = W.kernel
+    flag = torch.zeros(1, dtype=torch.int32, device=gpu)
+    for m_value, want in ((None, 0), (800.0, 1)):
+        loc = list(W.locals())
+        if m_value is not None:
+            loc[PARAMETER] = np.full_like(loc[PARAMETER], m_value)    # exp(800) overflows
+        for ij in ((ADJOINT, STATE), (ADJOINT, PARAMETER), (PARAMETER, PARAMETER)):
+            flag.zero_()
+            E = K.element_matrices(*ij, loc, precision=precision, nonfinite=flag)[0]
+            torch.cuda.synchronize()
+            assert int(flag.item()) == int(not bool(torch.isfinite(E).all())), (ij, m_value)
+            if ij == (ADJOINT, STATE):
+                assert int(flag.item()) == want, (route, precision, m_value)
+        if route == "sumfact" and precision != "bf16":
+            flag.zero_()
+            got = K.element_matrices_pair((ADJOINT, PARAMETER), (STATE, PARAMETER), loc,
+                                          precision=precision, nonfinite=flag)
+            torch.cuda.synchronize()
+            bad = not (bool(torch.isfinite(got[0]).all()) and bool(torch.isfinite(got[1]).all()))
+            assert int(flag.item()) == int(bad)
+    monkeypatch.delenv("HM_CONTRACTION")
+    prob = PDEVariationalProblem(W.spaces, W.density, W.batches, W.bc)
+    x = [W.u.copy(), W.m.copy(), W.p.copy()]
+    prob.set_linearization_point(x)
+    x[PARAMETER][:] = 800.0
+    with pytest.raises(RuntimeError):
+        prob.set_linearization_point(x)
+
+
 def test_batch_that_does_not_fit_raises_memory_error(gpu):
     """An element batch whose result does not fit raises MemoryError (the reference's
     message when no chunk fits, kernel.py:1096-1101): a result of 10^8 Q2 element matrices
