@@ -12,8 +12,13 @@
   two at a time from one element pass (``blocks``); FP64, both with the finiteness check of
   every element matrix; the two times and whether the values are the same bits.
 
+* ``contraction``: A's element matrices alone, on the sum-factorized route (vector units)
+  and on the dense matrix-core route (``v_mfma_f64_16x16x4f64`` / ``v_mfma_f32_16x16x4f32``),
+  FP64 and FP32: the time of a launch; the counters of the same launches give each route's
+  flop rate and matrix-core busy share against the peaks.
+
 ``python -m hippymfem_amd.configs precision --n 128`` / ``nonlinear --n 96`` /
-``linearization --n 64 [--nonlinear]``.
+``linearization --n 64 [--nonlinear]`` / ``contraction --n 64``.
 """
 
 import argparse
@@ -125,9 +130,39 @@ def linearization(n=64, nonlinear=False, steps=3):
     return line
 
 
+def contraction(n=64, steps=10):
+    import os
+
+    W = Workload(n)
+    loc = [torch.as_tensor(v).cuda() for v in W.locals()]
+    K = W.kernel
+    ne = K.group.ne
+    out = []
+    old = os.environ.get("HM_CONTRACTION")
+    try:
+        for route in ("sumfact", "mfma"):
+            os.environ["HM_CONTRACTION"] = route
+            for prec in ("fp64", "fp32"):
+                dt = torch.float64 if prec == "fp64" else torch.float32
+                E = torch.empty((ne, K.nd[ADJOINT], K.nd[STATE]), dtype=dt, device="cuda")
+                t = _time(lambda: K.element_matrices(ADJOINT, STATE, loc, precision=prec, out=E),
+                          steps)
+                line = {"n": n, "block": "A", "route": route, "precision": prec, "ms": t,
+                        "elements_per_s": ne / (t * 1e-3)}
+                print(json.dumps(line), flush=True)
+                out.append(line)
+                del E
+    finally:
+        if old is None:
+            os.environ.pop("HM_CONTRACTION", None)
+        else:
+            os.environ["HM_CONTRACTION"] = old
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__)
-    ap.add_argument("which", choices=["precision", "nonlinear", "linearization"])
+    ap.add_argument("which", choices=["precision", "nonlinear", "linearization", "contraction"])
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--nonlinear", action="store_true")
     ap.add_argument("--steps", type=int, default=3)
@@ -137,6 +172,8 @@ def main(argv=None):
         precision_form(a.n or 128, steps=a.steps)
     elif a.which == "linearization":
         linearization(a.n or 64, nonlinear=a.nonlinear, steps=a.steps)
+    elif a.which == "contraction":
+        contraction(a.n or 64, steps=a.steps)
     else:
         nonlinear(a.n or 96, steps=a.steps)
     print(json.dumps({"wall_s": time.perf_counter() - t0}), flush=True)
