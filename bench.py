@@ -235,6 +235,38 @@ def fp32_form(prob, loc, blk64, W, args, stream):
             "max_rel_err_vs_fp64": err_max, "fro_rel_err_vs_fp64": err_fro}
 
 
+def linearization_point(prob, loc, args):
+    """A and the Hessian blocks C, W_um, W_mm of the same problem at the seeded state,
+    parameter and adjoint (``set_linearization_point``; W_uu is zero for this linear density
+    and not assembled, as in the reference), FP64, with every block's elimination and
+    non-finite check; C and W_um from one element pass.  Host wall time a point."""
+    import gc
+
+    import torch
+
+    from hippymfem_amd import ADJOINT, PARAMETER, STATE
+
+    x = [loc[STATE], loc[PARAMETER], loc[ADJOINT]]
+    for _ in range(args.warmup):
+        got = prob.set_linearization_point(x)
+    torch.cuda.synchronize()
+    gc.collect()
+    gc.disable()
+    try:
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            got = prob.set_linearization_point(x)
+        torch.cuda.synchronize()
+        step = (time.perf_counter() - t0) / args.steps
+    finally:
+        gc.enable()
+    names = {(ADJOINT, STATE): "A", (ADJOINT, PARAMETER): "C", (STATE, PARAMETER): "W_um",
+             (PARAMETER, PARAMETER): "W_mm", (STATE, STATE): "W_uu"}
+    nnz = int(sum(prob.block_assembly(*ij).pattern.nnz for ij in got))
+    return {"blocks": [names[ij] for ij in got], "nnz": nnz, "ms_per_point": 1e3 * step,
+            "nnz_s": nnz / step}
+
+
 def copy_calibration():
     import torch
 
@@ -301,6 +333,7 @@ def main():
         "e2e": {"bytes": b_e2e, "GBs": b_e2e / step / 1e9,
                 "frac": b_e2e / step / 1e9 / HBM_PEAK_GBS},
     }
+    out["linearization_point"] = linearization_point(prob, loc, args)
     out["fp32_form"] = fp32_form(prob, loc, blk, W, args, stream)
     out["fp32_form"]["speedup_vs_fp64"] = step / (out["fp32_form"]["ms_per_step"] * 1e-3)
     if not args.no_serving:
