@@ -167,18 +167,9 @@ class QuadratureKernel:
         """Several blocks from one call (``element_matrices_many``, ``kernel.py:2048-2068``):
         blocks of one shape two at a time from one pass where the kernel allows, the others
         one by one."""
-        out, todo = {}, list(dict.fromkeys(tuple(ij) for ij in pairs))
-        while todo:
-            ij = todo.pop(0)
-            for k, other in enumerate(todo):
-                got = self.element_matrices_pair(ij, other, local_arrays, precision=precision)
-                if got is not None:
-                    out[ij], out[other] = [got[0]], [got[1]]
-                    del todo[k]
-                    break
-            else:
-                out[ij] = self.element_matrices(ij[0], ij[1], local_arrays, precision=precision)
-        return out
+        pairs = list(dict.fromkeys(tuple(ij) for ij in pairs))
+        got = self._many(pairs, local_arrays, precision, None)
+        return {ij: [got[ij]] for ij in pairs}
 
     def element_matrix_chunks(self, i, j, local_arrays, chunk=None, precision="fp64"):
         """A generator of ``(group, start, stop, array)`` chunks (``element_matrix_chunks``,
@@ -188,6 +179,34 @@ class QuadratureKernel:
             b = min(a + chunk, self.group.ne)
             yield 0, a, b, self.element_matrices(i, j, local_arrays, precision=precision,
                                                  elements=(a, b))[0]
+
+    def element_matrix_chunks_many(self, pairs, local_arrays, chunk=None, precision="fp64"):
+        """A generator of ``(group, start, stop, {(i, j): array})`` chunks of several blocks
+        (``element_matrix_chunks_many``, ``kernel.py:2026-2046``), each chunk's blocks as
+        ``element_matrices_many`` makes them (two of one shape from one pass), so the scatter
+        can take each chunk and nothing larger than a chunk is resident."""
+        chunk = int(chunk or ELEMENT_CHUNK or self.group.ne)
+        pairs = list(dict.fromkeys(tuple(ij) for ij in pairs))
+        for a in range(0, self.group.ne, chunk):
+            b = min(a + chunk, self.group.ne)
+            got = self._many(pairs, local_arrays, precision, (a, b))
+            yield 0, a, b, {ij: got[ij] for ij in pairs}
+
+    def _many(self, pairs, local_arrays, precision, elements):
+        out, todo = {}, list(pairs)
+        while todo:
+            ij = todo.pop(0)
+            for k, other in enumerate(todo):
+                got = self.element_matrices_pair(ij, other, local_arrays, precision=precision,
+                                                 elements=elements)
+                if got is not None:
+                    out[ij], out[other] = got
+                    del todo[k]
+                    break
+            else:
+                out[ij] = self.element_matrices(ij[0], ij[1], local_arrays, precision=precision,
+                                                elements=elements)[0]
+        return out
 
     def element_vectors(self, i, local_arrays, params=(), elements=None, out=None, stream=None):
         """``dR_e / d(slot i)`` ``[(ne, nd_i)]`` (``element_vectors``, ``kernel.py:2070-2078``)."""

@@ -181,7 +181,7 @@ class PDEVariationalProblem:
         ``kernel.py:2048-2068``)."""
         loc = self._locals(x)
         todo = list(dict.fromkeys(tuple(ij) for ij in pairs))
-        out = {}
+        passes = []
         while todo:
             ij = todo.pop(0)
             blk = self.block_assembly(ij[0], ij[1], precision)
@@ -196,16 +196,18 @@ class PDEVariationalProblem:
                     break
             else:
                 blk.element_matrices(loc)
-            # one flag for the pass (the first block's): a set flag is traced to its block
+            for d in done:
+                self.block_assembly(d[0], d[1], precision).scatter()
+            passes.append((blk, done))
+        # every pass launched before the first read of a flag (one flag a pass, the first
+        # block's); a set flag is traced to its block
+        for blk, done in passes:
             if not blk.finite():
                 for d in done:
                     require_finite_arrays([self.block_assembly(d[0], d[1], precision).E],
                                           "block (%d, %d)" % d)
-            for d in done:
-                b = self.block_assembly(d[0], d[1], precision)
-                b.scatter()
-                out[d] = b.finish()
-        return out
+        return {d: self.block_assembly(d[0], d[1], precision).finish()
+                for _, done in passes for d in done}
 
     def jacobian(self, x, precision="fp64"):
         """The forward Jacobian ``A`` (``_jacobian``, ``PDEVariationalProblem.py:542-659``):

@@ -351,6 +351,30 @@ def test_fused_chunks_equal_whole(gpu):
         assert np.array_equal(A.data.cpu().numpy(), whole), chunk
 
 
+@pytest.mark.parametrize("nonlinear", [False, True])
+def test_chunks_of_several_blocks(gpu, nonlinear):
+    """element_matrix_chunks_many (C and W_um, and at the nonlinear density A and W_uu, from
+    one pass a chunk) scattered chunk by chunk with scatter_many gives every block the bits of
+    its whole-batch assembly."""
+    from hippymfem_amd.csrassemble import finish_block, plan_block, scatter_many
+
+    W, _ = pair(4, perturb=0.2, nonlinear=nonlinear)
+    blocks = BLOCKS + ([(STATE, STATE)] if nonlinear else [])
+    plans = {}
+    for ij in blocks:
+        rule = ELIM[ij]
+        plans[ij] = plan_block(W.spaces[ij[0]], W.spaces[ij[1]], W.groups,
+                               test_ess=W.ess if rule["test"] else None,
+                               diag_policy=rule["diag_policy"], symmetric=rule["symmetric"])
+    for chunk in (9, 64):
+        chunks = W.kernel.element_matrix_chunks_many(blocks, W.locals(), chunk=chunk)
+        accs = scatter_many(plans, chunks)
+        for ij in blocks:
+            got = finish_block(plans[ij], accs[ij]).data.cpu().numpy()
+            want = assemble_gpu(W, *ij).data.cpu().numpy()
+            assert np.array_equal(got, want), (chunk, ij)
+
+
 def test_repeat_bit_identical(gpu):
     W, _ = pair(8)
     ref = assemble_gpu(W, ADJOINT, STATE).data.cpu().numpy()
