@@ -19,9 +19,9 @@ struct PointState {
   double uq;            // value of u
 };
 
-// The point state from the Jacobian of the element map J[d][k] = dx_d / dxi_k and the
-// fields' values and reference gradients at the point.
-__device__ inline void point_finish(const hm_problem& P, int q, const double J[3][3], double uq,
+// The point state from the Jacobian of the element map J[d][k] = dx_d / dxi_k, the point's
+// quadrature weight and the fields' values and reference gradients at the point.
+__device__ inline void point_finish(const hm_problem& P, double wq, const double J[3][3], double uq,
                                     const double ru[3], const double rp[3], double mq, PointState& S) {
   const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
   const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
@@ -38,7 +38,7 @@ __device__ inline void point_finish(const hm_problem& P, int q, const double J[3
   S.Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
   S.Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
   S.Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
-  S.w = P.wq[q] * fabs(det);
+  S.w = wq * fabs(det);
   for (int d = 0; d < 3; ++d) {
     S.gu[d] = S.Ji[0][d] * ru[0] + S.Ji[1][d] * ru[1] + S.Ji[2][d] * ru[2];
     S.gp[d] = S.Ji[0][d] * rp[0] + S.Ji[1][d] * rp[1] + S.Ji[2][d] * rp[2];
@@ -89,7 +89,7 @@ __device__ inline void point_state(const hm_problem& P, int q, const double* Xe,
     }
   }
   for (int a = 0; a < nm; ++a) mq = fma(Nm[a], me[a], mq);
-  point_finish(P, q, J, uq, ru, rp, mq, S);
+  point_finish(P, P.wq[q], J, uq, ru, rp, mq, S);
 }
 
 // The point state of element e at point q with every operand read from global memory as it
@@ -125,7 +125,7 @@ __device__ inline void point_state_stream(const hm_problem& P, int64_t e, int q,
   }
   const double* Nm = P.tab_m + (size_t)q * nm;
   for (int a = 0; a < nm; ++a) mq = fma(Nm[a], P.m[P.edofs_m[e * nm + a]], mq);
-  point_finish(P, q, J, uq, ru, rp, mq, S);
+  point_finish(P, P.wq[q], J, uq, ru, rp, mq, S);
 }
 
 // The pointwise block of test slot s and trial slot j, in the reference-gradient basis,

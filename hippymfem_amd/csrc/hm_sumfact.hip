@@ -84,7 +84,7 @@ __host__ __device__ inline int sf_element_words(int npair, int nd_u, int nd_m) {
 // at q1, then those times the 1-D factors at q2 and q3.
 __device__ inline void sf_point_state(const hm_problem& P, int q, const double* X, const double* U,
                                       const double* M, const double* Pv, bool fields, const double* TX,
-                                      const double* TM, const double* T1U, PointState& S) {
+                                      const double* TM, const double* TW, const double* T1U, PointState& S) {
   double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
 #pragma unroll 2
   for (int v = 0; v < 8; ++v)
@@ -132,15 +132,17 @@ __device__ inline void sf_point_state(const hm_problem& P, int q, const double* 
     }
   }
   for (int a = 0; a < nm; ++a) mq = fma(TM[a * 27 + q], M[a], mq);
-  point_finish(P, q, J, uq, ru, rp, mq, S);
+  point_finish(P, TW[q], J, uq, ru, rp, mq, S);
 }
 
 // LDS words of the tables every element reads: the trilinear map's gradients [27][8][3], the
-// parameter basis [27][nd_m] and, when the fields are read, the state space's 1-D basis
-// [2][n1_u][3] (room for n1_u up to 3).
+// parameter basis [27][nd_m], the quadrature weights [27] (read from LDS, not memory: a
+// memory read in the point phase would wait for the element-matrix stores issued before it,
+// as loads and stores share one counter) and, when the fields are read, the state space's
+// 1-D basis [2][n1_u][3] (room for n1_u up to 3).
 __host__ __device__ inline int sf_table_words(int nd_u, int nd_m, bool fields) {
   (void)nd_u;
-  const int w = 27 * 24 + 27 * nd_m + (fields ? 18 : 0);
+  const int w = 27 * 24 + 27 * nd_m + 27 + (fields ? 18 : 0);
   return (w + 1) & ~1;
 }
 
@@ -178,7 +180,8 @@ __attribute__((amdgpu_waves_per_eu(sf_waves<CT, NT, NC>(), sf_waves<CT, NT, NC>(
   const bool fields = SP.need_fields != 0;
   double* TX = lds;
   double* TM = TX + 27 * 24;
-  double* T1U = TM + 27 * nm;
+  double* TW = TM + 27 * nm;
+  double* T1U = TW + 27;
   {
     const int32_t* lxt = s == HM_PARAMETER ? P.lex_m : P.lex_u;
     const int32_t* lxc = j == HM_PARAMETER ? P.lex_m : P.lex_u;
@@ -191,6 +194,7 @@ __attribute__((amdgpu_waves_per_eu(sf_waves<CT, NT, NC>(), sf_waves<CT, NT, NC>(
     // the tables point-minor: entry (row, q) at row * 27 + q
     for (int i = threadIdx.x; i < 27 * 24; i += blockDim.x) TX[(i % 24) * 27 + i / 24] = P.tab_x[i];
     for (int i = threadIdx.x; i < 27 * nm; i += blockDim.x) TM[(i % nm) * 27 + i / nm] = P.tab_m[i];
+    for (int i = threadIdx.x; i < 27; i += blockDim.x) TW[i] = P.wq[i];
     if (fields) {
       for (int i = threadIdx.x; i < 2 * P.n1_u * 3; i += blockDim.x) T1U[i] = P.tab1_u[i];
       for (int i = threadIdx.x; i < nu; i += blockDim.x) UT[P.lex_u[i]] = i;
@@ -250,6 +254,11 @@ __attribute__((amdgpu_waves_per_eu(sf_waves<CT, NT, NC>(), sf_waves<CT, NT, NC>(
   };
   auto fetch_values = [&](int64_t el) {
     const bool ok = in_use && el < ne;
+    // one wait for the ids on every path (vmcnt(0) only: expcnt and lgkmcnt at their maxima),
+    // so that the coordinate and dof-value loads below leave together: waits placed by the
+    // compiler at each use of an id also waited for the loads issued just before them (after
+    // a store, loads and stores share the counter and it waits for all)
+    __builtin_amdgcn_s_waitcnt(0x0F70);
     if (ok && r < 8) {
       fx[0] = P.verts[(int64_t)ix * 3];
       fx[1] = P.verts[(int64_t)ix * 3 + 1];
@@ -342,7 +351,7 @@ __attribute__((amdgpu_waves_per_eu(sf_waves<CT, NT, NC>(), sf_waves<CT, NT, NC>(
     if (act)
       for (int q = r; q < 27; q += LPE) {
         PointState S;
-        sf_point_state(P, q, Xl, Ul, Ml, Pl, fields, TX, TM, T1U, S);
+        sf_point_state(P, q, Xl, Ul, Ml, Pl, fields, TX, TM, TW, T1U, S);
         double D[4][4];
         point_block(s, j, S, D);
         int p = 0;
