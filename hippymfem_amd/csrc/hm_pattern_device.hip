@@ -131,10 +131,13 @@ __global__ void k_pb_fill(const int32_t* __restrict__ tdofs, int64_t nrun, int64
   }
 }
 
-// Each row's run ids ascending (a row holds a few: insertion sort).
-__global__ void k_pb_sort_runs(const int32_t* __restrict__ rptr, int64_t nrow, int32_t* __restrict__ runs) {
+// Each row's run ids ascending (a row holds a few: insertion sort).  A row of more than
+// PB_MAXENT entries is refused by the build (k_pb_order flags it) and left unsorted: one thread
+// would otherwise sort a pathological row of tens of thousands of runs, for minutes.
+__global__ void k_pb_sort_runs(const int32_t* __restrict__ rptr, int64_t nrow, int nd_c, int32_t* __restrict__ runs) {
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrow; r += (int64_t)gridDim.x * blockDim.x) {
     const int32_t lo = rptr[r], hi = rptr[r + 1];
+    if ((int64_t)(hi - lo) * nd_c > PB_MAXENT) continue;
     for (int32_t i = lo + 1; i < hi; ++i) {
       const int32_t v = runs[i];
       int32_t j = i - 1;
@@ -264,7 +267,7 @@ extern "C" int hm_pattern_build_device(const int32_t* tdofs, const int32_t* cdof
   if (nrun > 0) {
     hipLaunchKernelGGL(k_pb_fill, dim3(grid_of(nrun, 256)), dim3(256), 0, st, tdofs, nrun, nrow, rptr, w.cursor,
                        runs);
-    hipLaunchKernelGGL(k_pb_sort_runs, dim3(grid_of(nrow, 256)), dim3(256), 0, st, rptr, nrow, runs);
+    hipLaunchKernelGGL(k_pb_sort_runs, dim3(grid_of(nrow, 256)), dim3(256), 0, st, rptr, nrow, nd_c, runs);
     const unsigned gr = (unsigned)(nrow < 1 ? 1 : (nrow < 262144 ? nrow : 262144));
     hipLaunchKernelGGL(k_pb_order, dim3(gr), dim3(64), 0, st, cdofs, nrow, nd_t, nd_c, diag_first, rptr, runs,
                        pos, w.cnt, w.flags);
