@@ -25,6 +25,7 @@ import numpy as np
 METRIC = ("assembled CSR nnz/s + % HBM roofline at 64³ P2 (FP64 vs FP32-MFMA); "
           "p50/p99 latency")
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6    # MI355X dense FP64 (matrix and vector)
 HERE = os.path.dirname(os.path.abspath(__file__))
 TRAFFIC_FILE = os.path.join(HERE, "profiles", "hbm_traffic_A_64cube.md")
 
@@ -306,10 +307,15 @@ def main():
     pat = blk.pattern
     b_elem, b_scatter = algorithmic_bytes(blk, W)
     traffic = measured_traffic()
+    # the contraction's algorithmic flops: the dense formulation E = Psi^T (w D) Psi of the
+    # stiffness-type block, 2 * 27 * 27 * 81 an element (the sum-factorized kernel does fewer)
+    flops = 2.0 * pat.nd_t * pat.nd_c * 81 * pat.ne
     kernels = {
         "k_sumfact": {"ms": 1e3 * t_kernel, "bytes": b_elem,
                       "GBs": b_elem / t_kernel / 1e9,
-                      "traffic": traffic.get("element_kernel")},
+                      "traffic": traffic.get("element_kernel"),
+                      "algorithmic_flops": flops, "TFLOPs": flops / t_kernel / 1e12,
+                      "frac_fp64_peak": flops / t_kernel / 1e12 / FP64_PEAK_TFLOPS},
         "k_scatter_rows": {"ms": 1e3 * t_scatter, "bytes": b_scatter,
                            "GBs": b_scatter / t_scatter / 1e9,
                            "traffic": traffic.get("scatter")},
