@@ -97,6 +97,47 @@ __global__ void __launch_bounds__(256) k_element_matrices(hm_problem P, int s, i
   }
 }
 
+// The element kernels of the vectors and the element-wise products give each wavefront
+// 64 / L elements, L = max(nq, nd_u, nd_m) lanes each.  The lanes of an element load its data
+// into the wavefront's own LDS region, act as its quadrature points (the pointwise weights
+// into LDS), then as its dofs (the sum over the points), with no block barrier in the loop.
+// The basis Psi[q][r][a] of the block's slots is in LDS for the whole block.
+struct WaveElement {
+  int L, EPW, slot, r;
+  bool in_use;
+};
+
+__device__ inline WaveElement wave_element(const hm_problem& P) {
+  WaveElement w;
+  w.L = P.nq;
+  if (P.nd_u > w.L) w.L = P.nd_u;
+  if (P.nd_m > w.L) w.L = P.nd_m;
+  w.EPW = 64 / w.L;
+  const int lane = threadIdx.x & 63;
+  w.slot = lane / w.L;
+  w.r = lane % w.L;
+  w.in_use = w.slot < w.EPW;
+  return w;
+}
+
+// The element's vertex coordinates and dof vectors into the wavefront's region, lane r
+// taking vertex r and dofs r.
+__device__ inline void load_element_lanes(const hm_problem& P, int64_t e, int r, double* Xe, double* ue,
+                                          double* me, double* pe) {
+  if (r < 8) {
+    const int32_t v = P.elem_verts[e * 8 + r];
+    Xe[r * 3] = P.verts[(size_t)v * 3];
+    Xe[r * 3 + 1] = P.verts[(size_t)v * 3 + 1];
+    Xe[r * 3 + 2] = P.verts[(size_t)v * 3 + 2];
+  }
+  if (r < P.nd_u) {
+    const int32_t d = P.edofs_u[e * P.nd_u + r];
+    ue[r] = P.u[d];
+    pe[r] = P.p[d];
+  }
+  if (r < P.nd_m) me[r] = P.m[P.edofs_m[e * P.nd_m + r]];
+}
+
 // Element vectors dR_e / d(slot i dofs): sum_q w [val N_a + (Jinv grad) . G_a].
 __global__ void __launch_bounds__(256) k_element_vectors(hm_problem P, int s, int64_t e0, int64_t e1,
                                                          double* out) {
@@ -105,17 +146,25 @@ __global__ void __launch_bounds__(256) k_element_vectors(hm_problem P, int s, in
   const int nd = s == HM_PARAMETER ? P.nd_m : P.nd_u;
   const double* tab = s == HM_PARAMETER ? P.tab_m : P.tab_u;
   double* Ps = lds;
-  double* W = Ps + nq * 4 * nd;  // W[q][4]: weights of the value and the 3 ref gradients
+  load_basis(tab, nq, nd, Ps);
+  __syncthreads();
+  const WaveElement w = wave_element(P);
+  const int words = nq * 4 + 24 + 2 * P.nd_u + P.nd_m;
+  double* W = Ps + nq * 4 * nd + (size_t)((threadIdx.x >> 6) * w.EPW + (w.in_use ? w.slot : 0)) * words;
   double* Xe = W + nq * 4;
   double* ue = Xe + 24;
   double* pe = ue + P.nd_u;
   double* me = pe + P.nd_u;
-  load_basis(tab, nq, nd, Ps);
-  for (int64_t e = e0 + blockIdx.x; e < e1; e += gridDim.x) {
-    __syncthreads();
-    load_element(P, e, Xe, ue, me, pe);
-    __syncthreads();
-    for (int q = threadIdx.x; q < nq; q += blockDim.x) {
+  const int64_t ne = e1 - e0;
+  const int64_t stride = (int64_t)gridDim.x * (blockDim.x >> 6) * w.EPW;
+  for (int64_t base = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * w.EPW; base < ne;
+       base += stride) {
+    const int64_t el = base + w.slot;
+    const bool act = w.in_use && el < ne;
+    if (act) load_element_lanes(P, e0 + el, w.r, Xe, ue, me, pe);
+    wave_sync();
+    if (act && w.r < nq) {
+      const int q = w.r;
       PointState S;
       point_state(P, q, Xe, ue, me, pe, S);
       double val = 0, gr[3] = {0, 0, 0};
@@ -132,13 +181,16 @@ __global__ void __launch_bounds__(256) k_element_vectors(hm_problem P, int s, in
       for (int k = 0; k < 3; ++k)
         W[q * 4 + 1 + k] = S.w * (S.Ji[k][0] * gr[0] + S.Ji[k][1] * gr[1] + S.Ji[k][2] * gr[2]);
     }
-    __syncthreads();
-    for (int a = threadIdx.x; a < nd; a += blockDim.x) {
+    wave_sync();
+    if (act && w.r < nd) {
+      const int a = w.r;
       double acc = 0;
       for (int q = 0; q < nq; ++q)
         for (int r = 0; r < 4; ++r) acc = fma(Ps[(q * 4 + r) * nd + a], W[q * 4 + r], acc);
-      out[(size_t)(e - e0) * nd + a] = acc;
+      out[(size_t)el * nd + a] = acc;
     }
+    // the region is rewritten by the next element
+    wave_sync();
   }
 }
 
@@ -153,22 +205,34 @@ __global__ void __launch_bounds__(256) k_element_hvp(hm_problem P, int s, int j,
   const double* tab_t = s == HM_PARAMETER ? P.tab_m : P.tab_u;
   const double* tab_c = j == HM_PARAMETER ? P.tab_m : P.tab_u;
   const int32_t* ed_c = j == HM_PARAMETER ? P.edofs_m : P.edofs_u;
+  const bool same = tab_t == tab_c;
   double* Pt = lds;
-  double* Pc = Pt + nq * 4 * nd_t;
-  double* W = Pc + nq * 4 * nd_c;
+  double* Pc = same ? Pt : Pt + nq * 4 * nd_t;
+  load_basis(tab_t, nq, nd_t, Pt);
+  if (!same) load_basis(tab_c, nq, nd_c, Pc);
+  __syncthreads();
+  const WaveElement w = wave_element(P);
+  const int words = nq * 4 + nd_c + 24 + 2 * P.nd_u + P.nd_m;
+  double* W = Pc + nq * 4 * nd_c + (size_t)((threadIdx.x >> 6) * w.EPW + (w.in_use ? w.slot : 0)) * words;
   double* de = W + nq * 4;
   double* Xe = de + nd_c;
   double* ue = Xe + 24;
   double* pe = ue + P.nd_u;
   double* me = pe + P.nd_u;
-  load_basis(tab_t, nq, nd_t, Pt);
-  load_basis(tab_c, nq, nd_c, Pc);
-  for (int64_t e = e0 + blockIdx.x; e < e1; e += gridDim.x) {
-    __syncthreads();
-    load_element(P, e, Xe, ue, me, pe);
-    for (int b = threadIdx.x; b < nd_c; b += blockDim.x) de[b] = dir[ed_c[e * nd_c + b]];
-    __syncthreads();
-    for (int q = threadIdx.x; q < nq; q += blockDim.x) {
+  const int64_t ne = e1 - e0;
+  const int64_t stride = (int64_t)gridDim.x * (blockDim.x >> 6) * w.EPW;
+  for (int64_t base = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * w.EPW; base < ne;
+       base += stride) {
+    const int64_t el = base + w.slot;
+    const bool act = w.in_use && el < ne;
+    if (act) {
+      const int64_t e = e0 + el;
+      load_element_lanes(P, e, w.r, Xe, ue, me, pe);
+      if (w.r < nd_c) de[w.r] = dir[ed_c[e * nd_c + w.r]];
+    }
+    wave_sync();
+    if (act && w.r < nq) {
+      const int q = w.r;
       PointState S;
       point_state(P, q, Xe, ue, me, pe, S);
       double Dq[4][4];
@@ -182,13 +246,15 @@ __global__ void __launch_bounds__(256) k_element_hvp(hm_problem P, int s, int j,
         W[q * 4 + r] = acc;
       }
     }
-    __syncthreads();
-    for (int a = threadIdx.x; a < nd_t; a += blockDim.x) {
+    wave_sync();
+    if (act && w.r < nd_t) {
+      const int a = w.r;
       double acc = 0;
       for (int q = 0; q < nq; ++q)
         for (int r = 0; r < 4; ++r) acc = fma(Pt[(q * 4 + r) * nd_t + a], W[q * 4 + r], acc);
-      out[(size_t)(e - e0) * nd_t + a] = acc;
+      out[(size_t)el * nd_t + a] = acc;
     }
+    wave_sync();
   }
 }
 
@@ -252,6 +318,30 @@ __global__ void __launch_bounds__(256) k_consistent_block(const float* __restric
       }
     }
   }
+}
+
+// Elements a wavefront of the vector and product kernels takes (wave_element on the device).
+static int wave_elements(const hm_problem& P) {
+  int L = P.nq;
+  if (P.nd_u > L) L = P.nd_u;
+  if (P.nd_m > L) L = P.nd_m;
+  return 64 / L;
+}
+
+// Dynamic LDS of a launch within the device's limit (the attribute set above 48 KiB).
+static int fit_lds(const void* kernel, size_t bytes) {
+  int limit = 64 * 1024, dev = 0;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0) limit = v;
+  }
+  HM_REQUIRE(bytes <= (size_t)limit, "element tables need %zu bytes of LDS, above the %d a block holds", bytes,
+             limit);
+  if (bytes > 48 * 1024) {
+    const hipError_t err = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (err != hipSuccess) HM_FAIL(HM_ERR_HIP, "hipFuncSetAttribute: %s", hipGetErrorString(err));
+  }
+  return HM_OK;
 }
 
 static int grid_for(int64_t n) {
@@ -345,10 +435,12 @@ extern "C" int hm_element_vectors(const hm_problem* P, int i, int64_t e0, int64_
   HM_REQUIRE(e0 >= 0 && e0 <= e1 && e1 <= P->ne, "elements outside the batch");
   if (e1 == e0) return HM_OK;
   const int nd = i == HM_PARAMETER ? P->nd_m : P->nd_u;
-  const size_t bytes = ((size_t)P->nq * 4 * nd + (size_t)P->nq * 4 + 24 + 2 * P->nd_u + P->nd_m) * 8;
-  HM_REQUIRE(bytes <= 64 * 1024, "element tables need %zu bytes of LDS, above 64 KiB", bytes);
-  hipLaunchKernelGGL(k_element_vectors, dim3(grid_for(e1 - e0)), dim3(256), bytes, (hipStream_t)stream,
-                     *P, i, e0, e1, out);
+  const int epb = 4 * wave_elements(*P);
+  const size_t bytes = ((size_t)P->nq * 4 * nd +
+                        (size_t)epb * ((size_t)P->nq * 4 + 24 + 2 * P->nd_u + P->nd_m)) * 8;
+  if (int rc = fit_lds((const void*)k_element_vectors, bytes)) return rc;
+  hipLaunchKernelGGL(k_element_vectors, dim3(grid_for((e1 - e0 + epb - 1) / epb)), dim3(256), bytes,
+                     (hipStream_t)stream, *P, i, e0, e1, out);
   return hip_check("hm_element_vectors");
 }
 
@@ -360,10 +452,12 @@ extern "C" int hm_element_hvp(const hm_problem* P, int i, int j, const double* d
   if (e1 == e0) return HM_OK;
   const int nd_t = i == HM_PARAMETER ? P->nd_m : P->nd_u;
   const int nd_c = j == HM_PARAMETER ? P->nd_m : P->nd_u;
-  const size_t bytes =
-      ((size_t)P->nq * 4 * (nd_t + nd_c) + (size_t)P->nq * 4 + nd_c + 24 + 2 * P->nd_u + P->nd_m) * 8;
-  HM_REQUIRE(bytes <= 64 * 1024, "element tables need %zu bytes of LDS, above 64 KiB", bytes);
-  hipLaunchKernelGGL(k_element_hvp, dim3(grid_for(e1 - e0)), dim3(256), bytes, (hipStream_t)stream, *P,
-                     i, j, dir, e0, e1, out);
+  const bool same = (i == HM_PARAMETER) == (j == HM_PARAMETER);
+  const int epb = 4 * wave_elements(*P);
+  const size_t bytes = ((size_t)P->nq * 4 * (nd_t + (same ? 0 : nd_c)) +
+                        (size_t)epb * ((size_t)P->nq * 4 + nd_c + 24 + 2 * P->nd_u + P->nd_m)) * 8;
+  if (int rc = fit_lds((const void*)k_element_hvp, bytes)) return rc;
+  hipLaunchKernelGGL(k_element_hvp, dim3(grid_for((e1 - e0 + epb - 1) / epb)), dim3(256), bytes,
+                     (hipStream_t)stream, *P, i, j, dir, e0, e1, out);
   return hip_check("hm_element_hvp");
 }

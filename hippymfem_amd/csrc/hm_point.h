@@ -9,6 +9,13 @@
 
 namespace hm {
 
+// Orders a wavefront's LDS writes before its later reads (the lanes of one wavefront share
+// element data in LDS; no block barrier).
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 // Everything one quadrature point of one element needs, in FP64.
 struct PointState {
   double w;           // w_q |det J|

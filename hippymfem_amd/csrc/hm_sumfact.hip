@@ -63,10 +63,6 @@ __device__ inline T quant(T x) {
 
 // The lanes of a wavefront see each other's LDS writes: a wavefront's LDS operations complete
 // in order, so a fence against compiler reordering and a wavefront barrier suffice.
-__device__ inline void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-}
 
 // LDS words of one element: its pairs W[npair][27], vertex coordinates [24], u and p dof
 // values [nd_u] each, m dof values [nd_m]; even, so every element starts 16-byte aligned.
