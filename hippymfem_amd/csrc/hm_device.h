@@ -29,6 +29,8 @@ int hm_element_matrices_sumfact(const hm_problem* P, int i, int j, int64_t e0, i
 // Two blocks of one shape from one pass of the sum-factorized kernel (hm_sumfact.hip).
 int hm_element_matrices_sumfact_pair(const hm_problem* P, int i1, int j1, int i2, int j2, int64_t e0,
                                      int64_t e1, int precision, void* out1, void* out2, void* stream);
+// Element vectors on tensor problems, sum-factorized point state (hm_sumfact.hip).
+int hm_element_vectors_sumfact(const hm_problem* P, int i, int64_t e0, int64_t e1, double* out, void* stream);
 // Fills n words of 4 or 8 bytes with zero on the stream (a kernel: safe under capture).
 int hm_fill_zero(void* p, size_t n, int word_bytes, hipStream_t st);
 // Sets *flag to 1 on the stream where one of the n words (fp64 for 8 bytes, fp32 for 4) is

@@ -434,6 +434,12 @@ extern "C" int hm_element_vectors(const hm_problem* P, int i, int64_t e0, int64_
   HM_REQUIRE(i >= 0 && i < 3, "slot %d outside 0..2", i);
   HM_REQUIRE(e0 >= 0 && e0 <= e1 && e1 <= P->ne, "elements outside the batch");
   if (e1 == e0) return HM_OK;
+  // tensor problems: the sum-factorized point state (the route of the element matrices)
+  const char* route = getenv("HM_CONTRACTION");
+  if (!(route && (route[0] == 'v' || route[0] == 'm'))) {
+    const int rc = hm_element_vectors_sumfact(P, i, e0, e1, out, stream);
+    if (rc != HM_ERR_UNSUPPORTED) return rc;
+  }
   const int nd = i == HM_PARAMETER ? P->nd_m : P->nd_u;
   const int epb = 4 * wave_elements(*P);
   const size_t bytes = ((size_t)P->nq * 4 * nd +

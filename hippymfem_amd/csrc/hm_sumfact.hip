@@ -586,6 +586,93 @@ static void plan_masks(SumfactPlan& SP) {
   }
 }
 
+// Element vectors dR_e / d(slot s dofs) = sum_q w [val N_a + (Jinv grad) . G_a] on tensor
+// problems: two elements a wavefront (27 lanes each), the lanes first the element's points (the
+// geometry and the fields by the sum-factorized point state, from the LDS tables of k_sumfact)
+// writing the pointwise weights into the wavefront's own LDS region, then the element's dofs
+// (the sum over the points against the basis, in LDS for the block).
+__global__ void __launch_bounds__(256) k_sumfact_vectors(const hm_problem P, int s, int64_t e0, int64_t e1,
+                                                         double* __restrict__ out) {
+  extern __shared__ double lds[];
+  __shared__ int32_t UT[32];
+  const int nu = P.nd_u, nm = P.nd_m;
+  const int nd = s == HM_PARAMETER ? nm : nu;
+  const double* tab = s == HM_PARAMETER ? P.tab_m : P.tab_u;
+  double* TX = lds;
+  double* TM = TX + 27 * 24;
+  double* TW = TM + 27 * nm;
+  double* T1U = TW + 27;
+  double* Ps = T1U + 18;  // Ps[q][r][a]: r = 0 the value, 1..3 the reference derivatives
+  for (int i = threadIdx.x; i < 27 * 24; i += blockDim.x) TX[(i % 24) * 27 + i / 24] = P.tab_x[i];
+  for (int i = threadIdx.x; i < 27 * nm; i += blockDim.x) TM[(i % nm) * 27 + i / nm] = P.tab_m[i];
+  for (int i = threadIdx.x; i < 27; i += blockDim.x) TW[i] = P.wq[i];
+  for (int i = threadIdx.x; i < 2 * P.n1_u * 3; i += blockDim.x) T1U[i] = P.tab1_u[i];
+  for (int i = threadIdx.x; i < nu; i += blockDim.x) UT[P.lex_u[i]] = i;
+  for (int i = threadIdx.x; i < 27 * 4 * nd; i += blockDim.x) {
+    const int a = i % nd, r = (i / nd) % 4, q = i / (4 * nd);
+    Ps[i] = r == 0 ? tab[q * nd + a] : tab[27 * nd + (q * nd + a) * 3 + (r - 1)];
+  }
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int slot = lane / 27, r = lane % 27;
+  const bool in_use = slot < 2;
+  const int words = 27 * 4 + 24 + 2 * nu + nm;
+  double* W = Ps + 27 * 4 * nd + (size_t)(wave * 2 + (in_use ? slot : 0)) * words;
+  double* X = W + 27 * 4;
+  double* U = X + 24;
+  double* Pv = U + nu;
+  double* M = Pv + nu;
+  const int64_t ne = e1 - e0;
+  const int64_t stride = (int64_t)gridDim.x * 8;
+  for (int64_t base = ((int64_t)blockIdx.x * 4 + wave) * 2; base < ne; base += stride) {
+    const int64_t el = base + slot;
+    const bool act = in_use && el < ne;
+    if (act) {
+      const int64_t e = e0 + el;
+      if (r < 8) {
+        const int32_t v = P.elem_verts[e * 8 + r];
+        X[r * 3] = P.verts[(size_t)v * 3];
+        X[r * 3 + 1] = P.verts[(size_t)v * 3 + 1];
+        X[r * 3 + 2] = P.verts[(size_t)v * 3 + 2];
+      }
+      if (r < nu) {
+        const int32_t d = P.edofs_u[e * nu + r];
+        U[UT[r]] = P.u[d];
+        Pv[UT[r]] = P.p[d];
+      }
+      if (r < nm) M[r] = P.m[P.edofs_m[e * nm + r]];
+    }
+    wave_sync();
+    if (act) {
+      const int q = r;
+      PointState S;
+      sf_point_state(P, q, X, U, M, Pv, true, TX, TM, TW, T1U, S);
+      double val = 0, gr[3] = {0, 0, 0};
+      if (s == HM_ADJOINT) {
+        val = -P.f;
+        for (int d = 0; d < 3; ++d) gr[d] = S.kap * S.g * S.gu[d];
+      } else if (s == HM_STATE) {
+        val = S.kap * S.g1 * S.dot;
+        for (int d = 0; d < 3; ++d) gr[d] = S.kap * S.g * S.gp[d];
+      } else {
+        val = S.kap * S.g * S.dot;
+      }
+      W[q * 4] = S.w * val;
+      for (int k = 0; k < 3; ++k)
+        W[q * 4 + 1 + k] = S.w * (S.Ji[k][0] * gr[0] + S.Ji[k][1] * gr[1] + S.Ji[k][2] * gr[2]);
+    }
+    wave_sync();
+    if (act && r < nd) {
+      double acc = 0;
+      for (int q = 0; q < 27; ++q)
+        for (int k = 0; k < 4; ++k) acc = fma(Ps[(q * 4 + k) * nd + r], W[q * 4 + k], acc);
+      out[(size_t)el * nd + r] = acc;
+    }
+    // the region is rewritten by the next element
+    wave_sync();
+  }
+}
+
 // the problem carries the tensor structure the sum-factorized contraction needs
 static int check_tensor(const hm_problem* P) {
   if (P->nq1 != 3 || P->nq != 27 || !P->tab1_u || !P->tab1_m || !P->lex_u || !P->lex_m)
@@ -667,4 +754,28 @@ int hm_element_matrices_sumfact_pair(const hm_problem* P, int i1, int j1, int i2
   if (precision == HM_FP64)
     return dispatch_orders<double, HM_FP64, 2>(*P, i1, j1, e0, e1, nt, nc, SP, out1, st, out2);
   return dispatch_orders<float, HM_FP32, 2>(*P, i1, j1, e0, e1, nt, nc, SP, out1, st, out2);
+}
+
+// Element vectors on tensor problems (k_sumfact_vectors); HM_ERR_UNSUPPORTED otherwise.
+int hm_element_vectors_sumfact(const hm_problem* P, int i, int64_t e0, int64_t e1, double* out, void* stream) {
+  if (int rc = check_tensor(P)) return rc;
+  if (P->n1_u < 2 || P->n1_u > 3) HM_FAIL(HM_ERR_UNSUPPORTED, "state fields of 1-D order outside 1..2");
+  const int nd = i == HM_PARAMETER ? P->nd_m : P->nd_u;
+  const size_t bytes = ((size_t)27 * 24 + 27 * P->nd_m + 27 + 18 + (size_t)27 * 4 * nd +
+                        (size_t)8 * (27 * 4 + 24 + 2 * P->nd_u + P->nd_m)) * sizeof(double);
+  int limit = 64 * 1024, dev = 0;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0) limit = v;
+  }
+  if (bytes > (size_t)limit) HM_FAIL(HM_ERR_UNSUPPORTED, "the element vectors need %zu bytes of LDS", bytes);
+  if (bytes > 48 * 1024) {
+    const hipError_t err = hipFuncSetAttribute((const void*)k_sumfact_vectors,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (err != hipSuccess) HM_FAIL(HM_ERR_HIP, "hipFuncSetAttribute: %s", hipGetErrorString(err));
+  }
+  const int64_t groups = (e1 - e0 + 7) / 8;
+  const unsigned grid = (unsigned)(groups < 65536 ? (groups < 1 ? 1 : groups) : 65536);
+  hipLaunchKernelGGL(k_sumfact_vectors, dim3(grid), dim3(256), bytes, (hipStream_t)stream, *P, i, e0, e1, out);
+  return hip_check("k_sumfact_vectors");
 }

@@ -74,12 +74,17 @@ def test_element_matrices_perturbed(gpu, perturb):
         assert rel(E, P.block(i, j)) <= TOL, (i, j)
 
 
+@pytest.mark.parametrize("order", [1, 2])
 @pytest.mark.parametrize("nonlinear", [False, True])
-def test_element_vectors(gpu, nonlinear):
-    W, P = pair(4, perturb=0.2, nonlinear=nonlinear)
-    for i in (STATE, PARAMETER, ADJOINT):
-        V = W.kernel.element_vectors(i, W.locals())[0].cpu().numpy()
-        assert rel(V, P.vector(i)) <= TOL, i
+def test_element_vectors(gpu, nonlinear, order, monkeypatch):
+    """The element vectors on the sum-factorized route (tensor problems) and on the generic
+    one agree with the oracle."""
+    W, P = pair(4, perturb=0.2, nonlinear=nonlinear, order=order)
+    for route in ("sumfact", "valu"):
+        monkeypatch.setenv("HM_CONTRACTION", route)
+        for i in (STATE, PARAMETER, ADJOINT):
+            V = W.kernel.element_vectors(i, W.locals())[0].cpu().numpy()
+            assert rel(V, P.vector(i)) <= TOL, (route, i)
 
 
 def test_element_hvp(gpu):
