@@ -87,6 +87,22 @@ def test_element_vectors(gpu, nonlinear, order, monkeypatch):
             assert rel(V, P.vector(i)) <= TOL, (route, i)
 
 
+def test_element_vector_and_product_ranges(gpu):
+    """Element vectors and products over a sub-range of the elements are that slice of the
+    whole batch, bit for bit (both routes of the vectors)."""
+    W, _ = pair(5, perturb=0.2, nonlinear=True)
+    K = W.kernel
+    d = np.random.default_rng(3).standard_normal(W.Vu.ndof)
+    for a, b in ((0, 1), (7, 61), (100, 125)):
+        for i in (STATE, PARAMETER, ADJOINT):
+            whole = K.element_vectors(i, W.locals())[0]
+            part = K.element_vectors(i, W.locals(), elements=(a, b))[0]
+            assert torch.equal(part, whole[a:b]), (a, b, i)
+        whole = K.element_hvp(ADJOINT, STATE, W.locals(), d)[0]
+        part = K.element_hvp(ADJOINT, STATE, W.locals(), d, elements=(a, b))[0]
+        assert torch.equal(part, whole[a:b]), (a, b)
+
+
 def test_element_hvp(gpu):
     W, P = pair(4, perturb=0.2, nonlinear=True)
     rng = np.random.default_rng(9)
