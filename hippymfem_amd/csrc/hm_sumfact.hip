@@ -56,9 +56,19 @@ constexpr int SF_WPB = 4;  // wavefronts a block
 // products and sums stay FP32 (a product of two such operands is exact in FP32).
 template <int QP, typename T>
 __device__ inline T quant(T x) {
-  if constexpr (QP == HM_BF16) return __bfloat162float(__float2bfloat16((float)x));
-  else if constexpr (QP == HM_FP16) return __half2float(__float2half((float)x));
-  else return x;
+  // the FP32 value is made opaque first: seen through, an FP64 source turned the hardware
+  // conversions into a software FP64 -> FP16 rounding of some thirty instructions
+  if constexpr (QP == HM_BF16) {
+    float f = (float)x;
+    asm("" : "+v"(f));
+    return __bfloat162float(__float2bfloat16(f));
+  } else if constexpr (QP == HM_FP16) {
+    float f = (float)x;
+    asm("" : "+v"(f));
+    return __half2float(__float2half(f));
+  } else {
+    return x;
+  }
 }
 
 // The lanes of a wavefront see each other's LDS writes: a wavefront's LDS operations complete
