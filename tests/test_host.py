@@ -63,6 +63,35 @@ def test_library_exports_every_declared_symbol():
     assert hm_lib.lib().hm_version() >= 1
 
 
+def test_ctypes_structs_match_header(tmp_path):
+    """The ctypes mirrors of hm_problem and hm_scatter_plan have the header's layout: every
+    field's offset and the size, as gcc lays the C structs out."""
+    structs = {"hm_problem": hm_lib.HmProblem, "hm_scatter_plan": hm_lib.HmScatterPlan}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "hippymfem_amd.h"',
+             "int main(void) {"]
+    for cname, cls in structs.items():
+        lines.append('printf("%%s size %%zu\\n", "%s", sizeof(%s));' % (cname, cname))
+        for f, _ in cls._fields_:
+            lines.append('printf("%%s %%s %%zu\\n", "%s", "%s", offsetof(%s, %s));'
+                         % (cname, f, cname, f))
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {}
+    for line in out:
+        if line:
+            parts = line.split()
+            got[(parts[0], parts[1])] = int(parts[2])
+    for cname, cls in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(cls), cname
+        for f, _ in cls._fields_:
+            assert got[(cname, f)] == getattr(cls, f).offset, (cname, f)
+
+
 @needs_lib
 @pytest.mark.parametrize("numbering", ["lattice", "mfem", "shuffled"])
 def test_product_host_matches_oracle(numbering):
