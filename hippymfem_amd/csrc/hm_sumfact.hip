@@ -534,8 +534,10 @@ static int launch_sumfact(const hm_problem& P, int i, int j, int64_t e0, int64_t
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (err != hipSuccess) HM_FAIL(HM_ERR_HIP, "hipFuncSetAttribute: %s", hipGetErrorString(err));
   }
-  // persistent: as many blocks as the device holds at once, each walking many elements (the
-  // tables are loaded once a block and the next element's data is in flight meanwhile)
+  // persistent: four times as many blocks as the device holds at once, each walking many
+  // elements (the tables are loaded once a block and the next element's data is in flight
+  // meanwhile); the later blocks fill the slots the first ones free, which balances the end
+  // of the launch (measured at 64^3: up to 3 % faster than one block a slot)
   int per_cu = 0, cus = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_sumfact<CT, NT, NC, QP, NB>, 256, bytes) !=
           hipSuccess ||
@@ -543,7 +545,7 @@ static int launch_sumfact(const hm_problem& P, int i, int j, int64_t e0, int64_t
     per_cu = 1;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
   const int64_t groups = (e1 - e0 + SF_WPB * EPW - 1) / (SF_WPB * EPW);
-  const int64_t full = (int64_t)per_cu * cus;
+  const int64_t full = (int64_t)per_cu * cus * 4;
   const int grid = (int)(groups < full ? groups : full);
   hipLaunchKernelGGL((k_sumfact<CT, NT, NC, QP, NB>), dim3(grid), dim3(256), bytes, st, P, i, j, e0, e1, SP,
                      (const double*)(i == HM_PARAMETER ? P.tab1_m : P.tab1_u),
