@@ -378,10 +378,13 @@ extern "C" int hm_element_matrices(const hm_problem* P, int i, int j, int64_t e0
   const size_t nent = (size_t)(e1 - e0) * nd_ti * nd_cj;
   if (nd_ti <= 32 && nd_cj <= 32 && (!valu || precision == HM_FP32)) {
     const int rc = hm_element_matrices_mfma(P, i, j, e0, e1, precision, out, stream);
-    if (rc) return rc;
-    return hm_flag_nonfinite(out, nent, precision == HM_FP64 ? 8 : 4, P->nonfinite, (hipStream_t)stream);
+    if (rc == HM_OK)
+      return hm_flag_nonfinite(out, nent, precision == HM_FP64 ? 8 : 4, P->nonfinite, (hipStream_t)stream);
+    if (rc != HM_ERR_UNSUPPORTED) return rc;
+    // the matrix-core tables do not fit (many points and parts): the generic contraction
   }
-  HM_REQUIRE(precision == HM_FP64, "the FP32 contraction needs at most 32 dofs per element");
+  // The generic contraction, FP64 arithmetic; FP32 results are its sums rounded once (the
+  // route of the single-precision forms where the others do not apply).
   const int nq = P->nq;
   const int nd_t = i == HM_PARAMETER ? P->nd_m : P->nd_u;
   const int nd_c = j == HM_PARAMETER ? P->nd_m : P->nd_u;
@@ -389,13 +392,19 @@ extern "C" int hm_element_matrices(const hm_problem* P, int i, int j, int64_t e0
   size_t words = (size_t)nq * 4 * nd_t + (same ? 0 : (size_t)nq * 4 * nd_c) + (size_t)nq * 4 * nd_c +
                  (size_t)nq * 16 + 24 + 2 * P->nd_u + P->nd_m;
   const size_t bytes = words * sizeof(double);
-  HM_REQUIRE(bytes <= 64 * 1024, "element tables need %zu bytes of LDS, above 64 KiB", bytes);
   const int parts = block_parts(i, j, P->density);
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_element_matrices<double>, dim3(grid_for(e1 - e0)), dim3(256), bytes, s, *P, i,
-                     j, e0, e1, parts, (double*)out);
+  if (precision == HM_FP64) {
+    if (int rc = fit_lds((const void*)k_element_matrices<double>, bytes)) return rc;
+    hipLaunchKernelGGL(k_element_matrices<double>, dim3(grid_for(e1 - e0)), dim3(256), bytes, s, *P, i,
+                       j, e0, e1, parts, (double*)out);
+  } else {
+    if (int rc = fit_lds((const void*)k_element_matrices<float>, bytes)) return rc;
+    hipLaunchKernelGGL(k_element_matrices<float>, dim3(grid_for(e1 - e0)), dim3(256), bytes, s, *P, i,
+                       j, e0, e1, parts, (float*)out);
+  }
   if (int rc = hip_check("hm_element_matrices")) return rc;
-  return hm_flag_nonfinite(out, nent, 8, P->nonfinite, s);
+  return hm_flag_nonfinite(out, nent, precision == HM_FP64 ? 8 : 4, P->nonfinite, s);
 }
 
 extern "C" int hm_element_matrices_pair(const hm_problem* P, int i1, int j1, int i2, int j2, int64_t e0,

@@ -74,20 +74,22 @@ __device__ inline T quant(T x) {
 // The lanes of a wavefront see each other's LDS writes: a wavefront's LDS operations complete
 // in order, so a fence against compiler reordering and a wavefront barrier suffice.
 
-// LDS words of one element: its pairs W[npair][27], vertex coordinates [24], u and p dof
+// LDS words of one element: its pairs W[npair][nq], vertex coordinates [24], u and p dof
 // values [nd_u] each, m dof values [nd_m]; even, so every element starts 16-byte aligned.
-__host__ __device__ inline int sf_element_words(int npair, int nd_u, int nd_m) {
-  const int w = npair * 27 + 24 + 2 * nd_u + nd_m;
+__host__ __device__ inline int sf_element_words(int npair, int nd_u, int nd_m, int nq) {
+  const int w = npair * nq + 24 + 2 * nd_u + nd_m;
   return (w + 1) & ~1;
 }
 
-// The point state of point q from LDS: the element's vertex coordinates X, dof values M, and
-// U, P in tensor order (a1 + n1 (a2 + n1 a3)); the tables stored point-minor (entry (row, q)
-// at row * 27 + q: lanes reading their own points hit consecutive words) -- TX[(v * 3 + k)][q],
-// TM[a][q] -- and the state space's 1-D basis T1U[2][n1][3] (values, then derivatives).
+// The point state of point q (NQ points a direction, q = q1 + NQ (q2 + NQ q3)) from LDS: the
+// element's vertex coordinates X, dof values M, and U, P in tensor order (a1 + n1 (a2 + n1 a3));
+// the tables stored point-minor (entry (row, q) at row * NQ^3 + q: lanes reading their own
+// points hit consecutive words) -- TX[(v * 3 + k)][q], TM[a][q] -- and the state space's 1-D
+// basis T1U[2][n1][NQ] (values, then derivatives).
 // The fields u, p and their reference gradients are summed one direction at a time: for each
 // (a2, a3) the sums over a1 of u and p against the value and the derivative of the 1-D basis
 // at q1, then those times the 1-D factors at q2 and q3.
+template <int NQ>
 __device__ inline void sf_point_state(const hm_problem& P, int q, const double* X, const double* U,
                                       const double* M, const double* Pv, bool fields, const double* TX,
                                       const double* TM, const double* TW, const double* T1U, PointState& S) {
@@ -96,7 +98,7 @@ __device__ inline void sf_point_state(const hm_problem& P, int q, const double* 
   for (int v = 0; v < 8; ++v)
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      const double g = TX[(v * 3 + k) * 27 + q];
+      const double g = TX[(v * 3 + k) * (NQ * NQ * NQ) + q];
       J[0][k] = fma(X[v * 3], g, J[0][k]);
       J[1][k] = fma(X[v * 3 + 1], g, J[1][k]);
       J[2][k] = fma(X[v * 3 + 2], g, J[2][k]);
@@ -105,22 +107,22 @@ __device__ inline void sf_point_state(const hm_problem& P, int q, const double* 
   double uq = 0, mq = 0, ru[3] = {0, 0, 0}, rp[3] = {0, 0, 0};
   if (fields) {
     const int n1 = P.n1_u;
-    const int q1 = q % 3, q2 = (q / 3) % 3, q3 = q / 9;
+    const int q1 = q % NQ, q2 = (q / NQ) % NQ, q3 = q / (NQ * NQ);
     // the 1-D factors are read from LDS as they are used (few registers: the point phase
     // shares the kernel's register budget with the contraction)
 #pragma unroll 1
     for (int a3 = 0; a3 < n1; ++a3) {
-      const double l3 = T1U[a3 * 3 + q3], d3 = T1U[(n1 + a3) * 3 + q3];
+      const double l3 = T1U[a3 * NQ + q3], d3 = T1U[(n1 + a3) * NQ + q3];
 #pragma unroll 1
       for (int a2 = 0; a2 < n1; ++a2) {
-        const double l2 = T1U[a2 * 3 + q2], d2 = T1U[(n1 + a2) * 3 + q2];
+        const double l2 = T1U[a2 * NQ + q2], d2 = T1U[(n1 + a2) * NQ + q2];
         const double* u = U + n1 * (a2 + n1 * a3);
         const double* pv = Pv + n1 * (a2 + n1 * a3);
         double svu = 0, sdu = 0, svp = 0, sdp = 0;
 #pragma unroll
         for (int a1 = 0; a1 < 3; ++a1) {
           if (a1 >= n1) break;
-          const double l1 = T1U[a1 * 3 + q1], d1 = T1U[(n1 + a1) * 3 + q1];
+          const double l1 = T1U[a1 * NQ + q1], d1 = T1U[(n1 + a1) * NQ + q1];
           svu = fma(l1, u[a1], svu);
           sdu = fma(d1, u[a1], sdu);
           svp = fma(l1, pv[a1], svp);
@@ -137,33 +139,34 @@ __device__ inline void sf_point_state(const hm_problem& P, int q, const double* 
       }
     }
   }
-  for (int a = 0; a < nm; ++a) mq = fma(TM[a * 27 + q], M[a], mq);
+  for (int a = 0; a < nm; ++a) mq = fma(TM[a * (NQ * NQ * NQ) + q], M[a], mq);
   point_finish(P, TW[q], J, uq, ru, rp, mq, S);
 }
 
-// LDS words of the tables every element reads: the trilinear map's gradients [27][8][3], the
-// parameter basis [27][nd_m], the quadrature weights [27] (read from LDS, not memory: a
-// memory read in the point phase would wait for the element-matrix stores issued before it,
-// as loads and stores share one counter) and, when the fields are read, the state space's
-// 1-D basis [2][n1_u][3] (room for n1_u up to 3).
-__host__ __device__ inline int sf_table_words(int nd_u, int nd_m, bool fields) {
+// LDS words of the tables every element reads (nq points, nq1 a direction): the trilinear
+// map's gradients [nq][8][3], the parameter basis [nq][nd_m], the quadrature weights [nq]
+// (read from LDS, not memory: a memory read in the point phase would wait for the
+// element-matrix stores issued before it, as loads and stores share one counter) and, when
+// the fields are read, the state space's 1-D basis [2][n1_u][nq1] (room for n1_u up to 3).
+__host__ __device__ inline int sf_table_words(int nd_u, int nd_m, bool fields, int nq1) {
   (void)nd_u;
-  const int w = 27 * 24 + 27 * nd_m + 27 + (fields ? 18 : 0);
+  const int nq = nq1 * nq1 * nq1;
+  const int w = nq * 24 + nq * nd_m + nq + (fields ? 6 * nq1 : 0);
   return (w + 1) & ~1;
 }
 
-template <typename CT, int NT, int NC>
+template <typename CT, int NT, int NC, int NQ>
 constexpr int sf_waves() {
-  return (sizeof(CT) == 4 && NT == 3 && NC == 3) ? 4 : 3;
+  return NQ > 3 ? 2 : (sizeof(CT) == 4 && NT == 3 && NC == 3) ? 4 : 3;
 }
 
 // NB = 2: the kernel writes two blocks of one shape (test and trial orders) from one pass over
 // the elements: geometry, fields and the pointwise pairs of both blocks once, then the two
 // contractions one after the other (the first block's matrix is stored at once, the second's
 // as the single block's is).
-template <typename CT, int NT, int NC, int QP, int NB>
+template <typename CT, int NT, int NC, int QP, int NB, int NQ>
 __global__ void __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(sf_waves<CT, NT, NC>(), sf_waves<CT, NT, NC>()))) k_sumfact(const hm_problem P, int s, int j, int64_t e0, int64_t e1,
+__attribute__((amdgpu_waves_per_eu(sf_waves<CT, NT, NC, NQ>(), sf_waves<CT, NT, NC, NQ>()))) k_sumfact(const hm_problem P, int s, int j, int64_t e0, int64_t e1,
                                                  SumfactPlan SP, const double* __restrict__ t1t,
                                                  const double* __restrict__ t1c, CT* __restrict__ out,
                                                  CT* __restrict__ out2) {
@@ -176,33 +179,34 @@ __attribute__((amdgpu_waves_per_eu(sf_waves<CT, NT, NC>(), sf_waves<CT, NT, NC>(
   constexpr int EPW = 64 / LPE;              // elements a wavefront
   constexpr int NPF = (27 + LPE - 1) / LPE;  // dofs a lane fetches
   constexpr int nd_t = NT * NT * NT, nd_c = NC * NC * NC;
+  constexpr int NQ2 = NQ * NQ, NQ3 = NQ2 * NQ;  // points a plane, points
   // the products of the 1-D tables, PT[dt][dc][a][b][q] = Xt_dt(a, q) Xc_dc(b, q), d the
   // derivative flags: every factor of stages 1, 2 and 3
-  __shared__ double PT[4 * NT * NC * 3];
+  __shared__ double PT[4 * NT * NC * NQ];
   __shared__ int32_t LX[nd_t + nd_c];  // local dof of each tensor index, test then trial
   __shared__ int32_t UT[32];            // tensor index of each local state dof
   extern __shared__ double lds[];
   const int npair = SP.npair, nu = P.nd_u, nm = P.nd_m;
   const bool fields = SP.need_fields != 0;
   double* TX = lds;
-  double* TM = TX + 27 * 24;
-  double* TW = TM + 27 * nm;
-  double* T1U = TW + 27;
+  double* TM = TX + NQ3 * 24;
+  double* TW = TM + NQ3 * nm;
+  double* T1U = TW + NQ3;
   {
     const int32_t* lxt = s == HM_PARAMETER ? P.lex_m : P.lex_u;
     const int32_t* lxc = j == HM_PARAMETER ? P.lex_m : P.lex_u;
     for (int i = threadIdx.x; i < nd_t; i += blockDim.x) LX[i] = lxt[i];
     for (int i = threadIdx.x; i < nd_c; i += blockDim.x) LX[nd_t + i] = lxc[i];
-    for (int i = threadIdx.x; i < 4 * NT * NC * 3; i += blockDim.x) {
-      const int q = i % 3, b = (i / 3) % NC, a = (i / (3 * NC)) % NT, dd = i / (3 * NC * NT);
-      PT[i] = t1t[((dd >> 1) * NT + a) * 3 + q] * t1c[((dd & 1) * NC + b) * 3 + q];
+    for (int i = threadIdx.x; i < 4 * NT * NC * NQ; i += blockDim.x) {
+      const int q = i % NQ, b = (i / NQ) % NC, a = (i / (NQ * NC)) % NT, dd = i / (NQ * NC * NT);
+      PT[i] = t1t[((dd >> 1) * NT + a) * NQ + q] * t1c[((dd & 1) * NC + b) * NQ + q];
     }
-    // the tables point-minor: entry (row, q) at row * 27 + q
-    for (int i = threadIdx.x; i < 27 * 24; i += blockDim.x) TX[(i % 24) * 27 + i / 24] = P.tab_x[i];
-    for (int i = threadIdx.x; i < 27 * nm; i += blockDim.x) TM[(i % nm) * 27 + i / nm] = P.tab_m[i];
-    for (int i = threadIdx.x; i < 27; i += blockDim.x) TW[i] = P.wq[i];
+    // the tables point-minor: entry (row, q) at row * NQ3 + q
+    for (int i = threadIdx.x; i < NQ3 * 24; i += blockDim.x) TX[(i % 24) * NQ3 + i / 24] = P.tab_x[i];
+    for (int i = threadIdx.x; i < NQ3 * nm; i += blockDim.x) TM[(i % nm) * NQ3 + i / nm] = P.tab_m[i];
+    for (int i = threadIdx.x; i < NQ3; i += blockDim.x) TW[i] = P.wq[i];
     if (fields) {
-      for (int i = threadIdx.x; i < 2 * P.n1_u * 3; i += blockDim.x) T1U[i] = P.tab1_u[i];
+      for (int i = threadIdx.x; i < 2 * P.n1_u * NQ; i += blockDim.x) T1U[i] = P.tab1_u[i];
       for (int i = threadIdx.x; i < nu; i += blockDim.x) UT[P.lex_u[i]] = i;
     }
   }
@@ -225,9 +229,9 @@ __attribute__((amdgpu_waves_per_eu(sf_waves<CT, NT, NC>(), sf_waves<CT, NT, NC>(
     a1 = r % NT;
     b1 = (r / NT) % NC;
     x2 = r / (NT * NC);
-    Wl = lds + sf_table_words(nu, nm, fields) +
-         (size_t)(wave * EPW + (in_use ? slot : 0)) * sf_element_words(npair, nu, nm);
-    Xl = Wl + npair * 27;
+    Wl = lds + sf_table_words(nu, nm, fields, NQ) +
+         (size_t)(wave * EPW + (in_use ? slot : 0)) * sf_element_words(npair, nu, nm, NQ3);
+    Xl = Wl + npair * NQ3;
     Ul = Xl + 24;
     Pl = Ul + nu;
     Ml = Pl + nu;
@@ -355,9 +359,9 @@ __attribute__((amdgpu_waves_per_eu(sf_waves<CT, NT, NC>(), sf_waves<CT, NT, NC>(
     wave_sync();
     // 2. lane = quadrature point: the pointwise pairs into LDS, in the order of k = 4 r + c
     if (act)
-      for (int q = r; q < 27; q += LPE) {
+      for (int q = r; q < NQ3; q += LPE) {
         PointState S;
-        sf_point_state(P, q, Xl, Ul, Ml, Pl, fields, TX, TM, TW, T1U, S);
+        sf_point_state<NQ>(P, q, Xl, Ul, Ml, Pl, fields, TX, TM, TW, T1U, S);
         double D[4][4];
         point_block(s, j, S, D);
         int p = 0;
@@ -365,7 +369,7 @@ __attribute__((amdgpu_waves_per_eu(sf_waves<CT, NT, NC>(), sf_waves<CT, NT, NC>(
         for (int k = 0; k < 16; ++k)
           if ((SP.kmask >> k) & 1) {
             const double v = D[k >> 2][k & 3];
-            Wl[p * 27 + q] = v;
+            Wl[p * NQ3 + q] = v;
             ++p;
           }
         if constexpr (NB == 2) {
@@ -373,7 +377,7 @@ __attribute__((amdgpu_waves_per_eu(sf_waves<CT, NT, NC>(), sf_waves<CT, NT, NC>(
 #pragma unroll
           for (int k = 0; k < 16; ++k)
             if ((SP.kmask2 >> k) & 1) {
-              Wl[p * 27 + q] = D[k >> 2][k & 3];
+              Wl[p * NQ3 + q] = D[k >> 2][k & 3];
               ++p;
             }
         }
@@ -388,9 +392,9 @@ __attribute__((amdgpu_waves_per_eu(sf_waves<CT, NT, NC>(), sf_waves<CT, NT, NC>(
     if constexpr (LOWP) {
       double m0 = 0.0, m1 = 0.0;
       if (act && r < npair)
-        for (int q = 0; q < 27; ++q) m0 = fmax(m0, fabs(Wl[r * 27 + q]));
+        for (int q = 0; q < NQ3; ++q) m0 = fmax(m0, fabs(Wl[r * NQ3 + q]));
       if (act && LPE + r < npair)
-        for (int q = 0; q < 27; ++q) m1 = fmax(m1, fabs(Wl[(LPE + r) * 27 + q]));
+        for (int q = 0; q < NQ3; ++q) m1 = fmax(m1, fabs(Wl[(LPE + r) * NQ3 + q]));
       top = 0.0;
       for (int p = 0; p < npair; ++p) {
         const double v0 = __shfl(m0, slot * LPE + p % LPE, 64), v1 = __shfl(m1, slot * LPE + p % LPE, 64);
@@ -417,86 +421,140 @@ __attribute__((amdgpu_waves_per_eu(sf_waves<CT, NT, NC>(), sf_waves<CT, NT, NC>(
     for (int t = 0; t < 4; ++t) {
       const uint32_t tm = live & (uint32_t)SP.tmask[t] & bm;
       if (!act || tm == 0) continue;
-      CT s2[N2][3];
+      CT s2[N2][NQ];
 #pragma unroll
-      for (int i = 0; i < N2; ++i) s2[i][0] = s2[i][1] = s2[i][2] = 0;
+      for (int i = 0; i < N2; ++i)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) s2[i][q] = 0;
       for (int p = 0; p < npair; ++p) {
         if (!((tm >> p) & 1)) continue;
         const int d1 = ((SP.pr[p] == 1) << 1) | (SP.pc[p] == 1);
         const int d2 = ((SP.pr[p] == 2) << 1) | (SP.pc[p] == 2);
         // every LDS operand of the pair in flight at once (one wait, not one a use)
-        const double* wv = Wl + p * 27;
-        const double* f1 = PT + ((d1 * NT + a1) * NC + b1) * 3;
-        const double g0d = f1[0], g1d = f1[1], g2d = f1[2];
-        double hd[N2][3];
+        const double* wv = Wl + p * NQ3;
+        const double* f1 = PT + ((d1 * NT + a1) * NC + b1) * NQ;
+        double gd[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) gd[q] = f1[q];
+        double hd[N2][NQ];
 #pragma unroll
         for (int i = 0; i < N2; ++i) {
-          const double* f2 = PT + ((d2 * NT + a2_of(i)) * NC + b2_of(i)) * 3;
-          hd[i][0] = f2[0];
-          hd[i][1] = f2[1];
-          hd[i][2] = f2[2];
+          const double* f2 = PT + ((d2 * NT + a2_of(i)) * NC + b2_of(i)) * NQ;
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) hd[i][q] = f2[q];
         }
         __builtin_amdgcn_sched_barrier(0);
-        const CT g0 = quant<QP>((CT)g0d), g1 = quant<QP>((CT)g1d), g2 = quant<QP>((CT)g2d);
+        CT g[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) g[q] = quant<QP>((CT)gd[q]);
         // one q3 plane at a time:
         //   stage 1: s1[q2] = sum_q1 X(a1, q1) Y(b1, q1) W(q1, q2, q3)
         //   stage 2: s2[i][q3] += sum_q2 X(a2, q2) Y(b2, q2) s1[q2], (a2, b2) of i
 #pragma unroll
-        for (int q3 = 0; q3 < 3; ++q3) {
-          CT s1[3];
+        for (int q3 = 0; q3 < NQ; ++q3) {
+          CT s1[NQ];
 #pragma unroll
-          for (int q2 = 0; q2 < 3; ++q2) {
-            const double* wr = wv + 3 * q2 + 9 * q3;
-            if constexpr (LOWP) {
+          for (int q2 = 0; q2 < NQ; ++q2) {
+            const double* wr = wv + NQ * q2 + NQ2 * q3;
+            if constexpr (LOWP && NQ == 3) {
               const CT w0 = quant<QP>((CT)(wr[0] * wscale)), w1 = quant<QP>((CT)(wr[1] * wscale)),
                        w2 = quant<QP>((CT)(wr[2] * wscale));
-              s1[q2] = quant<QP>(fma(g2, w2, fma(g1, w1, g0 * w0)));
+              s1[q2] = quant<QP>(fma(g[2], w2, fma(g[1], w1, g[0] * w0)));
+            } else if constexpr (LOWP) {
+              CT w[NQ];
+#pragma unroll
+              for (int q1 = 0; q1 < NQ; ++q1) w[q1] = quant<QP>((CT)(wr[q1] * wscale));
+              CT a = g[0] * w[0];
+#pragma unroll
+              for (int q1 = 1; q1 < NQ; ++q1) a = fma(g[q1], w[q1], a);
+              s1[q2] = quant<QP>(a);
+            } else if constexpr (NQ == 3) {
+              s1[q2] = fma(g[2], (CT)wr[2], fma(g[1], (CT)wr[1], g[0] * (CT)wr[0]));
             } else {
-              s1[q2] = fma(g2, (CT)wr[2], fma(g1, (CT)wr[1], g0 * (CT)wr[0]));
+              CT a = g[0] * (CT)wr[0];
+#pragma unroll
+              for (int q1 = 1; q1 < NQ; ++q1) a = fma(g[q1], (CT)wr[q1], a);
+              s1[q2] = a;
             }
           }
+          if constexpr (NQ == 3) {
 #pragma unroll
-          for (int i = 0; i < N2; ++i)
-            s2[i][q3] = fma(quant<QP>((CT)hd[i][2]), s1[2],
-                            fma(quant<QP>((CT)hd[i][1]), s1[1], fma(quant<QP>((CT)hd[i][0]), s1[0], s2[i][q3])));
+            for (int i = 0; i < N2; ++i)
+              s2[i][q3] = fma(quant<QP>((CT)hd[i][2]), s1[2],
+                              fma(quant<QP>((CT)hd[i][1]), s1[1], fma(quant<QP>((CT)hd[i][0]), s1[0], s2[i][q3])));
+          } else {
+#pragma unroll
+            for (int i = 0; i < N2; ++i) {
+              CT a = s2[i][q3];
+#pragma unroll
+              for (int q2 = 0; q2 < NQ; ++q2) a = fma(quant<QP>((CT)hd[i][q2]), s1[q2], a);
+              s2[i][q3] = a;
+            }
+          }
         }
       }
       // stage 3 for this type of third-direction factors, the 1-D rows read as wavefront-uniform
       // values from memory (scalar loads):
       //   E[i][a3][b3] += sum_q3 Xt(a3, q3) (Yc(b3, q3) s2[i][q3])
-      const double* bt = t1t + (t >> 1) * NT * 3;
-      const double* bc = t1c + (t & 1) * NC * 3;
+      const double* bt = t1t + (t >> 1) * NT * NQ;
+      const double* bc = t1c + (t & 1) * NC * NQ;
       if constexpr (LOWP) {
         // the stage-3 operands: s2 and the table products, each rounded
 #pragma unroll
         for (int i = 0; i < N2; ++i)
 #pragma unroll
-          for (int q = 0; q < 3; ++q) s2[i][q] = quant<QP>(s2[i][q]);
+          for (int q = 0; q < NQ; ++q) s2[i][q] = quant<QP>(s2[i][q]);
 #pragma unroll
         for (int a3 = 0; a3 < NT; ++a3)
 #pragma unroll
           for (int b3 = 0; b3 < NC; ++b3) {
-            const double* f3 = PT + ((t * NT + a3) * NC + b3) * 3;
-            const CT k0 = quant<QP>((CT)f3[0]), k1 = quant<QP>((CT)f3[1]), k2 = quant<QP>((CT)f3[2]);
+            const double* f3 = PT + ((t * NT + a3) * NC + b3) * NQ;
+            if constexpr (NQ == 3) {
+              const CT k0 = quant<QP>((CT)f3[0]), k1 = quant<QP>((CT)f3[1]), k2 = quant<QP>((CT)f3[2]);
 #pragma unroll
-            for (int i = 0; i < N2; ++i)
-              acc[i][a3][b3] = fma(k2, s2[i][2], fma(k1, s2[i][1], fma(k0, s2[i][0], acc[i][a3][b3])));
+              for (int i = 0; i < N2; ++i)
+                acc[i][a3][b3] = fma(k2, s2[i][2], fma(k1, s2[i][1], fma(k0, s2[i][0], acc[i][a3][b3])));
+            } else {
+              CT k[NQ];
+#pragma unroll
+              for (int q = 0; q < NQ; ++q) k[q] = quant<QP>((CT)f3[q]);
+#pragma unroll
+              for (int i = 0; i < N2; ++i) {
+                CT a = acc[i][a3][b3];
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) a = fma(k[q], s2[i][q], a);
+                acc[i][a3][b3] = a;
+              }
+            }
           }
         continue;
       }
 #pragma unroll
       for (int b3 = 0; b3 < NC; ++b3) {
-        CT u[N2][3];
+        CT u[N2][NQ];
 #pragma unroll
         for (int i = 0; i < N2; ++i)
 #pragma unroll
-          for (int q = 0; q < 3; ++q) u[i][q] = (CT)bc[b3 * 3 + q] * s2[i][q];
+          for (int q = 0; q < NQ; ++q) u[i][q] = (CT)bc[b3 * NQ + q] * s2[i][q];
 #pragma unroll
         for (int a3 = 0; a3 < NT; ++a3) {
-          const CT k0 = (CT)bt[a3 * 3], k1 = (CT)bt[a3 * 3 + 1], k2 = (CT)bt[a3 * 3 + 2];
+          if constexpr (NQ == 3) {
+            const CT k0 = (CT)bt[a3 * 3], k1 = (CT)bt[a3 * 3 + 1], k2 = (CT)bt[a3 * 3 + 2];
 #pragma unroll
-          for (int i = 0; i < N2; ++i)
-            acc[i][a3][b3] = fma(k2, u[i][2], fma(k1, u[i][1], fma(k0, u[i][0], acc[i][a3][b3])));
+            for (int i = 0; i < N2; ++i)
+              acc[i][a3][b3] = fma(k2, u[i][2], fma(k1, u[i][1], fma(k0, u[i][0], acc[i][a3][b3])));
+          } else {
+            CT k[NQ];
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) k[q] = (CT)bt[a3 * NQ + q];
+#pragma unroll
+            for (int i = 0; i < N2; ++i) {
+              CT a = acc[i][a3][b3];
+#pragma unroll
+              for (int q = 0; q < NQ; ++q) a = fma(k[q], u[i][q], a);
+              acc[i][a3][b3] = a;
+            }
+          }
         }
       }
     }
@@ -516,21 +574,22 @@ __attribute__((amdgpu_waves_per_eu(sf_waves<CT, NT, NC>(), sf_waves<CT, NT, NC>(
   if (bad && P.nonfinite) atomicOr(P.nonfinite, 1);
 }
 
-template <typename CT, int NT, int NC, int QP, int NB>
+template <typename CT, int NT, int NC, int QP, int NB, int NQ>
 static int launch_sumfact(const hm_problem& P, int i, int j, int64_t e0, int64_t e1, const SumfactPlan& SP,
                           void* out, void* out2, hipStream_t st) {
   constexpr int EPW = 64 / (NT * NC * (NC >= NT ? NC : NT));
-  const size_t bytes = ((size_t)sf_table_words(P.nd_u, P.nd_m, SP.need_fields != 0) +
-                        (size_t)SF_WPB * EPW * sf_element_words(SP.npair, P.nd_u, P.nd_m)) * sizeof(double);
+  const size_t bytes = ((size_t)sf_table_words(P.nd_u, P.nd_m, SP.need_fields != 0, NQ) +
+                        (size_t)SF_WPB * EPW * sf_element_words(SP.npair, P.nd_u, P.nd_m, NQ * NQ * NQ)) *
+                       sizeof(double);
   int limit = 64 * 1024, dev = 0;
   if (hipGetDevice(&dev) == hipSuccess) {
     int v = 0;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0) limit = v;
   }
-  if (bytes + 4 * NT * NC * 3 * sizeof(double) > (size_t)limit)
+  if (bytes + 4 * NT * NC * NQ * sizeof(double) > (size_t)limit)
     HM_FAIL(HM_ERR_UNSUPPORTED, "the sum-factorized contraction needs %zu bytes of LDS", bytes);
   if (bytes > 48 * 1024) {
-    hipError_t err = hipFuncSetAttribute((const void*)k_sumfact<CT, NT, NC, QP, NB>,
+    hipError_t err = hipFuncSetAttribute((const void*)k_sumfact<CT, NT, NC, QP, NB, NQ>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (err != hipSuccess) HM_FAIL(HM_ERR_HIP, "hipFuncSetAttribute: %s", hipGetErrorString(err));
   }
@@ -539,7 +598,7 @@ static int launch_sumfact(const hm_problem& P, int i, int j, int64_t e0, int64_t
   // meanwhile); the later blocks fill the slots the first ones free, which balances the end
   // of the launch (measured at 64^3: up to 3 % faster than one block a slot)
   int per_cu = 0, cus = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_sumfact<CT, NT, NC, QP, NB>, 256, bytes) !=
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_sumfact<CT, NT, NC, QP, NB, NQ>, 256, bytes) !=
           hipSuccess ||
       per_cu < 1)
     per_cu = 1;
@@ -547,7 +606,7 @@ static int launch_sumfact(const hm_problem& P, int i, int j, int64_t e0, int64_t
   const int64_t groups = (e1 - e0 + SF_WPB * EPW - 1) / (SF_WPB * EPW);
   const int64_t full = (int64_t)per_cu * cus * 4;
   const int grid = (int)(groups < full ? groups : full);
-  hipLaunchKernelGGL((k_sumfact<CT, NT, NC, QP, NB>), dim3(grid), dim3(256), bytes, st, P, i, j, e0, e1, SP,
+  hipLaunchKernelGGL((k_sumfact<CT, NT, NC, QP, NB, NQ>), dim3(grid), dim3(256), bytes, st, P, i, j, e0, e1, SP,
                      (const double*)(i == HM_PARAMETER ? P.tab1_m : P.tab1_u),
                      (const double*)(j == HM_PARAMETER ? P.tab1_m : P.tab1_u), (CT*)out, (CT*)out2);
   return hip_check("k_sumfact");
@@ -556,10 +615,17 @@ static int launch_sumfact(const hm_problem& P, int i, int j, int64_t e0, int64_t
 template <typename CT, int QP, int NB = 1>
 static int dispatch_orders(const hm_problem& P, int i, int j, int64_t e0, int64_t e1, int nt, int nc,
                            const SumfactPlan& SP, void* out, hipStream_t st, void* out2 = nullptr) {
-  if (nt == 3 && nc == 3) return launch_sumfact<CT, 3, 3, QP, NB>(P, i, j, e0, e1, SP, out, out2, st);
-  if (nt == 3 && nc == 2) return launch_sumfact<CT, 3, 2, QP, NB>(P, i, j, e0, e1, SP, out, out2, st);
-  if (nt == 2 && nc == 3) return launch_sumfact<CT, 2, 3, QP, NB>(P, i, j, e0, e1, SP, out, out2, st);
-  return launch_sumfact<CT, 2, 2, QP, NB>(P, i, j, e0, e1, SP, out, out2, st);
+  // 3 points a direction (the 3 x 3 x 3 rule) or 4 (4 x 4 x 4, degree 6 or 7)
+  if (P.nq1 == 4) {
+    if (nt == 3 && nc == 3) return launch_sumfact<CT, 3, 3, QP, NB, 4>(P, i, j, e0, e1, SP, out, out2, st);
+    if (nt == 3 && nc == 2) return launch_sumfact<CT, 3, 2, QP, NB, 4>(P, i, j, e0, e1, SP, out, out2, st);
+    if (nt == 2 && nc == 3) return launch_sumfact<CT, 2, 3, QP, NB, 4>(P, i, j, e0, e1, SP, out, out2, st);
+    return launch_sumfact<CT, 2, 2, QP, NB, 4>(P, i, j, e0, e1, SP, out, out2, st);
+  }
+  if (nt == 3 && nc == 3) return launch_sumfact<CT, 3, 3, QP, NB, 3>(P, i, j, e0, e1, SP, out, out2, st);
+  if (nt == 3 && nc == 2) return launch_sumfact<CT, 3, 2, QP, NB, 3>(P, i, j, e0, e1, SP, out, out2, st);
+  if (nt == 2 && nc == 3) return launch_sumfact<CT, 2, 3, QP, NB, 3>(P, i, j, e0, e1, SP, out, out2, st);
+  return launch_sumfact<CT, 2, 2, QP, NB, 3>(P, i, j, e0, e1, SP, out, out2, st);
 }
 
 // The nonzero (r, c) pairs of block (i, j), r, c in {value, d/dxi_1, d/dxi_2, d/dxi_3}: the
@@ -599,44 +665,47 @@ static void plan_masks(SumfactPlan& SP) {
 }
 
 // Element vectors dR_e / d(slot s dofs) = sum_q w [val N_a + (Jinv grad) . G_a] on tensor
-// problems: two elements a wavefront (27 lanes each), the lanes first the element's points (the
+// problems: 64 / NQ^3 elements a wavefront (NQ^3 lanes each: two for the 3 x 3 x 3 rule, one
+// for 4 x 4 x 4), the lanes first the element's points (the
 // geometry and the fields by the sum-factorized point state, from the LDS tables of k_sumfact)
 // writing the pointwise weights into the wavefront's own LDS region, then the element's dofs
 // (the sum over the points against the basis, in LDS for the block).
+template <int NQ>
 __global__ void __launch_bounds__(256) k_sumfact_vectors(const hm_problem P, int s, int64_t e0, int64_t e1,
                                                          double* __restrict__ out) {
+  constexpr int NQ3 = NQ * NQ * NQ, EPW = 64 / NQ3;
   extern __shared__ double lds[];
   __shared__ int32_t UT[32];
   const int nu = P.nd_u, nm = P.nd_m;
   const int nd = s == HM_PARAMETER ? nm : nu;
   const double* tab = s == HM_PARAMETER ? P.tab_m : P.tab_u;
   double* TX = lds;
-  double* TM = TX + 27 * 24;
-  double* TW = TM + 27 * nm;
-  double* T1U = TW + 27;
-  double* Ps = T1U + 18;  // Ps[q][r][a]: r = 0 the value, 1..3 the reference derivatives
-  for (int i = threadIdx.x; i < 27 * 24; i += blockDim.x) TX[(i % 24) * 27 + i / 24] = P.tab_x[i];
-  for (int i = threadIdx.x; i < 27 * nm; i += blockDim.x) TM[(i % nm) * 27 + i / nm] = P.tab_m[i];
-  for (int i = threadIdx.x; i < 27; i += blockDim.x) TW[i] = P.wq[i];
-  for (int i = threadIdx.x; i < 2 * P.n1_u * 3; i += blockDim.x) T1U[i] = P.tab1_u[i];
+  double* TM = TX + NQ3 * 24;
+  double* TW = TM + NQ3 * nm;
+  double* T1U = TW + NQ3;
+  double* Ps = T1U + 6 * NQ;  // Ps[q][r][a]: r = 0 the value, 1..3 the reference derivatives
+  for (int i = threadIdx.x; i < NQ3 * 24; i += blockDim.x) TX[(i % 24) * NQ3 + i / 24] = P.tab_x[i];
+  for (int i = threadIdx.x; i < NQ3 * nm; i += blockDim.x) TM[(i % nm) * NQ3 + i / nm] = P.tab_m[i];
+  for (int i = threadIdx.x; i < NQ3; i += blockDim.x) TW[i] = P.wq[i];
+  for (int i = threadIdx.x; i < 2 * P.n1_u * NQ; i += blockDim.x) T1U[i] = P.tab1_u[i];
   for (int i = threadIdx.x; i < nu; i += blockDim.x) UT[P.lex_u[i]] = i;
-  for (int i = threadIdx.x; i < 27 * 4 * nd; i += blockDim.x) {
+  for (int i = threadIdx.x; i < NQ3 * 4 * nd; i += blockDim.x) {
     const int a = i % nd, r = (i / nd) % 4, q = i / (4 * nd);
-    Ps[i] = r == 0 ? tab[q * nd + a] : tab[27 * nd + (q * nd + a) * 3 + (r - 1)];
+    Ps[i] = r == 0 ? tab[q * nd + a] : tab[NQ3 * nd + (q * nd + a) * 3 + (r - 1)];
   }
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int slot = lane / 27, r = lane % 27;
-  const bool in_use = slot < 2;
-  const int words = 27 * 4 + 24 + 2 * nu + nm;
-  double* W = Ps + 27 * 4 * nd + (size_t)(wave * 2 + (in_use ? slot : 0)) * words;
-  double* X = W + 27 * 4;
+  const int slot = lane / NQ3, r = lane % NQ3;
+  const bool in_use = slot < EPW;
+  const int words = NQ3 * 4 + 24 + 2 * nu + nm;
+  double* W = Ps + NQ3 * 4 * nd + (size_t)(wave * EPW + (in_use ? slot : 0)) * words;
+  double* X = W + NQ3 * 4;
   double* U = X + 24;
   double* Pv = U + nu;
   double* M = Pv + nu;
   const int64_t ne = e1 - e0;
-  const int64_t stride = (int64_t)gridDim.x * 8;
-  for (int64_t base = ((int64_t)blockIdx.x * 4 + wave) * 2; base < ne; base += stride) {
+  const int64_t stride = (int64_t)gridDim.x * 4 * EPW;
+  for (int64_t base = ((int64_t)blockIdx.x * 4 + wave) * EPW; base < ne; base += stride) {
     const int64_t el = base + slot;
     const bool act = in_use && el < ne;
     if (act) {
@@ -658,7 +727,7 @@ __global__ void __launch_bounds__(256) k_sumfact_vectors(const hm_problem P, int
     if (act) {
       const int q = r;
       PointState S;
-      sf_point_state(P, q, X, U, M, Pv, true, TX, TM, TW, T1U, S);
+      sf_point_state<NQ>(P, q, X, U, M, Pv, true, TX, TM, TW, T1U, S);
       double val = 0, gr[3] = {0, 0, 0};
       if (s == HM_ADJOINT) {
         val = -P.f;
@@ -676,7 +745,7 @@ __global__ void __launch_bounds__(256) k_sumfact_vectors(const hm_problem P, int
     wave_sync();
     if (act && r < nd) {
       double acc = 0;
-      for (int q = 0; q < 27; ++q)
+      for (int q = 0; q < NQ3; ++q)
         for (int k = 0; k < 4; ++k) acc = fma(Ps[(q * 4 + k) * nd + r], W[q * 4 + k], acc);
       out[(size_t)el * nd + r] = acc;
     }
@@ -687,7 +756,8 @@ __global__ void __launch_bounds__(256) k_sumfact_vectors(const hm_problem P, int
 
 // the problem carries the tensor structure the sum-factorized contraction needs
 static int check_tensor(const hm_problem* P) {
-  if (P->nq1 != 3 || P->nq != 27 || !P->tab1_u || !P->tab1_m || !P->lex_u || !P->lex_m)
+  if ((P->nq1 != 3 && P->nq1 != 4) || P->nq != P->nq1 * P->nq1 * P->nq1 || !P->tab1_u || !P->tab1_m ||
+      !P->lex_u || !P->lex_m)
     HM_FAIL(HM_ERR_UNSUPPORTED, "no tensor structure for the sum-factorized contraction");
   if (P->nd_u > 32 || P->nd_m > 32) HM_FAIL(HM_ERR_UNSUPPORTED, "more than 32 dofs an element");
   if (P->nd_u != P->n1_u * P->n1_u * P->n1_u || P->nd_m != P->n1_m * P->n1_m * P->n1_m)
@@ -773,8 +843,10 @@ int hm_element_vectors_sumfact(const hm_problem* P, int i, int64_t e0, int64_t e
   if (int rc = check_tensor(P)) return rc;
   if (P->n1_u < 2 || P->n1_u > 3) HM_FAIL(HM_ERR_UNSUPPORTED, "state fields of 1-D order outside 1..2");
   const int nd = i == HM_PARAMETER ? P->nd_m : P->nd_u;
-  const size_t bytes = ((size_t)27 * 24 + 27 * P->nd_m + 27 + 18 + (size_t)27 * 4 * nd +
-                        (size_t)8 * (27 * 4 + 24 + 2 * P->nd_u + P->nd_m)) * sizeof(double);
+  const int nq1 = P->nq1, nq = P->nq, epb = 4 * (64 / nq);
+  const size_t bytes = ((size_t)nq * 24 + (size_t)nq * P->nd_m + nq + 6 * nq1 + (size_t)nq * 4 * nd +
+                        (size_t)epb * (nq * 4 + 24 + 2 * P->nd_u + P->nd_m)) * sizeof(double);
+  const void* kern = nq1 == 4 ? (const void*)k_sumfact_vectors<4> : (const void*)k_sumfact_vectors<3>;
   int limit = 64 * 1024, dev = 0;
   if (hipGetDevice(&dev) == hipSuccess) {
     int v = 0;
@@ -782,12 +854,14 @@ int hm_element_vectors_sumfact(const hm_problem* P, int i, int64_t e0, int64_t e
   }
   if (bytes > (size_t)limit) HM_FAIL(HM_ERR_UNSUPPORTED, "the element vectors need %zu bytes of LDS", bytes);
   if (bytes > 48 * 1024) {
-    const hipError_t err = hipFuncSetAttribute((const void*)k_sumfact_vectors,
-                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    const hipError_t err = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (err != hipSuccess) HM_FAIL(HM_ERR_HIP, "hipFuncSetAttribute: %s", hipGetErrorString(err));
   }
-  const int64_t groups = (e1 - e0 + 7) / 8;
+  const int64_t groups = (e1 - e0 + epb - 1) / epb;
   const unsigned grid = (unsigned)(groups < 65536 ? (groups < 1 ? 1 : groups) : 65536);
-  hipLaunchKernelGGL(k_sumfact_vectors, dim3(grid), dim3(256), bytes, (hipStream_t)stream, *P, i, e0, e1, out);
+  if (nq1 == 4)
+    hipLaunchKernelGGL(k_sumfact_vectors<4>, dim3(grid), dim3(256), bytes, (hipStream_t)stream, *P, i, e0, e1, out);
+  else
+    hipLaunchKernelGGL(k_sumfact_vectors<3>, dim3(grid), dim3(256), bytes, (hipStream_t)stream, *P, i, e0, e1, out);
   return hip_check("k_sumfact_vectors");
 }

@@ -529,8 +529,36 @@ def test_lowp_contraction_error(gpu, precision):
         assert err > 0.0                          # the operands were rounded
 
 
+@pytest.mark.parametrize("nonlinear", [False, True])
+def test_default_quadrature_degree(gpu, nonlinear, monkeypatch):
+    """The reference's default rule for P2 (degree 6, 4 x 4 x 4 points), sum-factorized with
+    four points a direction: element matrices (FP64 within 1e-13, FP32 within 1e-6, BF16 and
+    FP16 within their bounds) and element vectors against the oracle on the same rule; the
+    matrix-core route agrees too."""
+    W = Workload(3, perturb=0.2, nonlinear=nonlinear, quadrature_degree=6)
+    P = O.Problem(3, perturb=0.2, density="nonlinear" if nonlinear else "linear", n1d=4)
+    assert W.kernel.group.nq == 64
+    blocks = BLOCKS + ([(STATE, STATE)] if nonlinear else [])
+    for i, j in blocks:
+        Eo = P.block(i, j)
+        E = W.kernel.element_matrices(i, j, W.locals())[0].cpu().numpy()
+        assert rel(E, Eo) <= TOL, (i, j, rel(E, Eo))
+        E32 = W.kernel.element_matrices(i, j, W.locals(), precision="fp32")[0].cpu().numpy()
+        assert rel(E32, Eo) <= 1e-6, (i, j, rel(E32, Eo))
+        for lowp in ("bf16", "fp16"):
+            El = W.kernel.element_matrices(i, j, W.locals(), precision=lowp)[0].cpu().numpy()
+            assert rel(El, Eo) <= LOWP_BOUND[lowp], (lowp, i, j, rel(El, Eo))
+    for i in (STATE, PARAMETER, ADJOINT):
+        V = W.kernel.element_vectors(i, W.locals())[0].cpu().numpy()
+        assert rel(V, P.vector(i)) <= TOL, i
+    monkeypatch.setenv("HM_CONTRACTION", "mfma")
+    for i, j in blocks:
+        E = W.kernel.element_matrices(i, j, W.locals())[0].cpu().numpy()
+        assert rel(E, P.block(i, j)) <= TOL, ("mfma", i, j)
+
+
 def test_lowp_needs_tensor_structure(gpu):
-    W = Workload(2, quadrature_degree=6)            # 4 points a direction: no sum factorization
+    W = Workload(2, quadrature_degree=2)            # 2 points a direction: no sum factorization
     with pytest.raises(NotImplementedError):
         W.kernel.element_matrices(ADJOINT, STATE, W.locals(), precision="bf16")
 

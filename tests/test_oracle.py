@@ -32,6 +32,25 @@ def test_gauss_rule_closed_form():
         assert abs(approx - 1.0 / ((a + 1) * (b + 1) * (c + 1))) < 1e-15
 
 
+def test_four_point_rule():
+    """The 4-point rule (degree 6, the default for P2 fields): the closed-form Gauss-Legendre
+    nodes 1/2 -+ 1/2 sqrt(3/7 +- 2/7 sqrt(6/5)) and weights (18 -+ sqrt 30)/72 on [0, 1]; exact
+    through degree 7 in each variable and not at degree 8."""
+    x, w = O.gauss_legendre_01(4)
+    r = np.sqrt(6.0 / 5.0)
+    inner, outer = np.sqrt(3 / 7 - 2 / 7 * r), np.sqrt(3 / 7 + 2 / 7 * r)
+    assert np.allclose(x, [0.5 - 0.5 * outer, 0.5 - 0.5 * inner, 0.5 + 0.5 * inner, 0.5 + 0.5 * outer],
+                       atol=1e-15, rtol=0)
+    c = np.sqrt(30.0)
+    assert np.allclose(w, [(18 - c) / 72, (18 + c) / 72, (18 + c) / 72, (18 - c) / 72], atol=1e-15, rtol=0)
+    pts, wts = O.integration_rule(4)
+    assert pts.shape == (64, 3)
+    for a, b, cc in [(7, 0, 0), (2, 7, 4), (7, 7, 7), (6, 1, 5)]:
+        approx = np.sum(wts * pts[:, 0] ** a * pts[:, 1] ** b * pts[:, 2] ** cc)
+        assert abs(approx - 1.0 / ((a + 1) * (b + 1) * (cc + 1))) < 1e-15
+    assert abs(np.sum(wts * pts[:, 0] ** 8) - 1.0 / 9) > 1e-6
+
+
 # ------------------------------------------------------------------------ basis
 @pytest.mark.parametrize("order", [1, 2])
 @pytest.mark.parametrize("local", ["lattice", "mfem"])
