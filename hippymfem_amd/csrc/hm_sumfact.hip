@@ -10,7 +10,7 @@
 // diffusion density it takes 17.5k multiply-adds an element, or 7k where the element is
 // axis-aligned, against 59k for the dense contraction.
 //
-// A wavefront holds 64 / (NT NC max(NT, NC)) elements.  The lanes of an element first stage its vertex
+// A wavefront holds 64 / (NT NC max(NT, NC)) elements (at most 3 with 4 points a direction).  The lanes of an element first stage its vertex
 // coordinates and dof values in LDS, then act as its quadrature points and write the block's
 // pointwise pairs W[p][q] (D^ of pair p at point q, FP64) to LDS, then act as its contraction
 // tasks (a1, b1, x2), x2 = b2 or a2: each runs stages 1 and 2 in registers and stage 3 into its
@@ -155,6 +155,15 @@ __host__ __device__ inline int sf_table_words(int nd_u, int nd_m, bool fields, i
   return (w + 1) & ~1;
 }
 
+// Elements a wavefront: as many as its 64 lanes hold contraction tasks (NT NC max(NT, NC) an
+// element), at most 3 with 4 points a direction, whose pointwise pairs take 64 LDS words each
+// (8 P1 x P1 elements a wavefront would need more LDS than a compute unit has)
+template <int NT, int NC, int NQ>
+constexpr int sf_epw() {
+  constexpr int lpe = NT * NC * (NC >= NT ? NC : NT);
+  return (NQ > 3 && 64 / lpe > 3) ? 3 : 64 / lpe;
+}
+
 template <typename CT, int NT, int NC, int NQ>
 constexpr int sf_waves() {
   return NQ > 3 ? 2 : (sizeof(CT) == 4 && NT == 3 && NC == 3) ? 4 : 3;
@@ -176,7 +185,7 @@ __attribute__((amdgpu_waves_per_eu(sf_waves<CT, NT, NC, NQ>(), sf_waves<CT, NT, 
   constexpr bool BMAP = NC >= NT;
   constexpr int N2 = BMAP ? NT : NC;          // the stage-2 index held in registers
   constexpr int LPE = NT * NC * (BMAP ? NC : NT);  // lanes an element
-  constexpr int EPW = 64 / LPE;              // elements a wavefront
+  constexpr int EPW = sf_epw<NT, NC, NQ>();  // elements a wavefront
   constexpr int NPF = (27 + LPE - 1) / LPE;  // dofs a lane fetches
   constexpr int nd_t = NT * NT * NT, nd_c = NC * NC * NC;
   constexpr int NQ2 = NQ * NQ, NQ3 = NQ2 * NQ;  // points a plane, points
@@ -577,7 +586,7 @@ __attribute__((amdgpu_waves_per_eu(sf_waves<CT, NT, NC, NQ>(), sf_waves<CT, NT, 
 template <typename CT, int NT, int NC, int QP, int NB, int NQ>
 static int launch_sumfact(const hm_problem& P, int i, int j, int64_t e0, int64_t e1, const SumfactPlan& SP,
                           void* out, void* out2, hipStream_t st) {
-  constexpr int EPW = 64 / (NT * NC * (NC >= NT ? NC : NT));
+  constexpr int EPW = sf_epw<NT, NC, NQ>();
   const size_t bytes = ((size_t)sf_table_words(P.nd_u, P.nd_m, SP.need_fields != 0, NQ) +
                         (size_t)SF_WPB * EPW * sf_element_words(SP.npair, P.nd_u, P.nd_m, NQ * NQ * NQ)) *
                        sizeof(double);

@@ -74,7 +74,9 @@ typedef struct hm_problem {
   const double* m;
   const double* p;
   /* Tensor-product structure, for the sum-factorized contraction (nq1 = 0: none).
-   *   nq1          Gauss points per direction (nq = nq1^3, point (i, j, k) at i + nq1 (j + nq1 k))
+   *   nq1          Gauss points per direction (nq = nq1^3, point (i, j, k) at i + nq1 (j + nq1 k));
+   *                the sum-factorized kernels take 3 or 4 (degree 4 or 5, and 6 or 7), other
+   *                rules run on the generic kernels
    *   n1_u, n1_m   1-D nodes per direction of the two spaces (order + 1)
    *   tab1_u/_m    1-D basis at the 1-D points: [2][n1][nq1], values then derivatives
    *   lex_u/_m     local dof of the tensor index a1 + n1 (a2 + n1 a3), (n1^3,) int32 */

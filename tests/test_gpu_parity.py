@@ -529,14 +529,16 @@ def test_lowp_contraction_error(gpu, precision):
         assert err > 0.0                          # the operands were rounded
 
 
+@pytest.mark.parametrize("order", [1, 2])
 @pytest.mark.parametrize("nonlinear", [False, True])
-def test_default_quadrature_degree(gpu, nonlinear, monkeypatch):
+def test_default_quadrature_degree(gpu, nonlinear, order, monkeypatch):
     """The reference's default rule for P2 (degree 6, 4 x 4 x 4 points), sum-factorized with
     four points a direction: element matrices (FP64 within 1e-13, FP32 within 1e-6, BF16 and
     FP16 within their bounds) and element vectors against the oracle on the same rule; the
-    matrix-core route agrees too."""
-    W = Workload(3, perturb=0.2, nonlinear=nonlinear, quadrature_degree=6)
-    P = O.Problem(3, perturb=0.2, density="nonlinear" if nonlinear else "linear", n1d=4)
+    matrix-core route agrees too.  P1 state fields on the same rule take the four-point
+    kernels with two nodes a direction."""
+    W = Workload(3, order=order, perturb=0.2, nonlinear=nonlinear, quadrature_degree=6)
+    P = O.Problem(3, order=order, perturb=0.2, density="nonlinear" if nonlinear else "linear", n1d=4)
     assert W.kernel.group.nq == 64
     blocks = BLOCKS + ([(STATE, STATE)] if nonlinear else [])
     for i, j in blocks:
