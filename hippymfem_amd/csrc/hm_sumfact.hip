@@ -367,30 +367,46 @@ __attribute__((amdgpu_waves_per_eu(sf_waves<CT, NT, NC, NQ>(), sf_waves<CT, NT, 
     fetch_ids(el + stride);
     wave_sync();
     // 2. lane = quadrature point: the pointwise pairs into LDS, in the order of k = 4 r + c
-    if (act)
-      for (int q = r; q < NQ3; q += LPE) {
-        PointState S;
-        sf_point_state<NQ>(P, q, Xl, Ul, Ml, Pl, fields, TX, TM, TW, T1U, S);
-        double D[4][4];
-        point_block(s, j, S, D);
-        int p = 0;
+    auto point_task = [&](int q, double* We, const double* Xe, const double* Ue, const double* Me,
+                          const double* Pe) {
+      PointState S;
+      sf_point_state<NQ>(P, q, Xe, Ue, Me, Pe, fields, TX, TM, TW, T1U, S);
+      double D[4][4];
+      point_block(s, j, S, D);
+      int p = 0;
+#pragma unroll
+      for (int k = 0; k < 16; ++k)
+        if ((SP.kmask >> k) & 1) {
+          const double v = D[k >> 2][k & 3];
+          We[p * NQ3 + q] = v;
+          ++p;
+        }
+      if constexpr (NB == 2) {
+        point_block(SP.s2, SP.j2, S, D);
 #pragma unroll
         for (int k = 0; k < 16; ++k)
-          if ((SP.kmask >> k) & 1) {
-            const double v = D[k >> 2][k & 3];
-            Wl[p * NQ3 + q] = v;
+          if ((SP.kmask2 >> k) & 1) {
+            We[p * NQ3 + q] = D[k >> 2][k & 3];
             ++p;
           }
-        if constexpr (NB == 2) {
-          point_block(SP.s2, SP.j2, S, D);
-#pragma unroll
-          for (int k = 0; k < 16; ++k)
-            if ((SP.kmask2 >> k) & 1) {
-              Wl[p * NQ3 + q] = D[k >> 2][k & 3];
-              ++p;
-            }
+      }
+    };
+    if constexpr (NQ > 3) {
+      // more points than an element's lanes: every lane of the wavefront takes the next
+      // (element, point) of its elements, so no round leaves lanes idle but the last
+      const int words = sf_element_words(npair, nu, nm, NQ3);
+      double* W0 = lds + sf_table_words(nu, nm, fields, NQ) + (size_t)wave * EPW * words;
+      for (int t = lane; t < EPW * NQ3; t += 64) {
+        const int ee = t / NQ3, q = t - ee * NQ3;
+        if (base + ee < ne) {
+          double* We = W0 + (size_t)ee * words;
+          const double* Xe = We + npair * NQ3;
+          point_task(q, We, Xe, Xe + 24, Xe + 24 + 2 * nu, Xe + 24 + nu);
         }
       }
+    } else if (act) {
+      for (int q = r; q < NQ3; q += LPE) point_task(q, Wl, Xl, Ul, Ml, Pl);
+    }
     wave_sync();
     // 3. the pairs this element needs.  The low-precision forms also measure every pair
     // against the element's largest pointwise weight, their scale, and leave out the pairs
