@@ -358,14 +358,23 @@ def reference_product(cache, req, precision="fp64"):
 
 
 def summary(requests, wall=None):
+    """Throughput and latency percentiles of a served stream; ``by_model`` splits the
+    latencies by problem (``"64^3 P2"`` ...), largest first: the largest products (17 MB of
+    result at 64^3 P2) set the tail of a mixed stream."""
     lat = np.array([r.latency for r in requests if r.latency is not None])
     done = [r for r in requests if r.done is not None]
     span = wall if wall is not None else (max(r.done for r in done) - min(r.arrival for r in done))
     nnz = sum(r.model.pattern.nnz for r in done if r.model is not None)
+    groups = {}
+    for r in done:
+        groups.setdefault((r.n, r.order), []).append(r.latency)
+    by_model = {"%d^3 P%d" % key: {"requests": len(v), "p50_ms": 1e3 * float(np.percentile(v, 50)),
+                                   "p99_ms": 1e3 * float(np.percentile(v, 99))}
+                for key, v in sorted(groups.items(), key=lambda kv: (-kv[0][0], -kv[0][1]))}
     return {"requests": len(done), "throughput_req_s": len(done) / span,
             "throughput_nnz_s": nnz / span,
             "p50_ms": 1e3 * float(np.percentile(lat, 50)), "p99_ms": 1e3 * float(np.percentile(lat, 99)),
-            "mean_ms": 1e3 * float(lat.mean()), "max_ms": 1e3 * float(lat.max())}
+            "mean_ms": 1e3 * float(lat.mean()), "max_ms": 1e3 * float(lat.max()), "by_model": by_model}
 
 
 def error_vs_fp64(cache, requests, count=8):

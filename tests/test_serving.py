@@ -9,7 +9,7 @@ budget.
 import numpy as np
 import pytest
 
-from hippymfem_amd.serving import SIZES, poisson_requests
+from hippymfem_amd.serving import SIZES, poisson_requests, summary
 
 
 def test_poisson_requests():
@@ -21,6 +21,23 @@ def test_poisson_requests():
     assert abs(np.mean([r.order == 2 for r in reqs]) - 0.5) < 0.05
     again = poisson_requests(2000, 50.0, seed=3)
     assert [(r.n, r.order, r.seed) for r in again] == [(r.n, r.order, r.seed) for r in reqs]
+
+
+def test_summary_by_model():
+    """Percentiles over the stream and per problem, from the requests' own clocks."""
+    reqs = poisson_requests(400, 100.0, seed=4)
+    for k, r in enumerate(reqs):
+        r.done = r.arrival + 1e-3 * r.n / 8 * (1.0 + (k % 7) / 70)     # latency grows with n
+    out = summary(reqs)
+    lat = np.array([r.done - r.arrival for r in reqs])
+    assert out["requests"] == 400
+    assert abs(out["p99_ms"] - 1e3 * np.percentile(lat, 99)) < 1e-9
+    assert sum(v["requests"] for v in out["by_model"].values()) == 400
+    keys = list(out["by_model"])
+    assert keys[0] == "64^3 P2" and keys[-1] == "8^3 P1"
+    big = [r.done - r.arrival for r in reqs if (r.n, r.order) == (64, 2)]
+    assert abs(out["by_model"]["64^3 P2"]["p50_ms"] - 1e3 * np.percentile(big, 50)) < 1e-9
+    assert out["by_model"]["64^3 P2"]["p99_ms"] > out["by_model"]["8^3 P1"]["p99_ms"]
 
 
 @pytest.mark.gpu
