@@ -553,6 +553,11 @@ def test_default_quadrature_degree(gpu, nonlinear, order, monkeypatch):
     for i in (STATE, PARAMETER, ADJOINT):
         V = W.kernel.element_vectors(i, W.locals())[0].cpu().numpy()
         assert rel(V, P.vector(i)) <= TOL, i
+    # element ranges that split the wavefronts' element groups: the same bits as the whole pass
+    full = W.kernel.element_matrices(ADJOINT, STATE, W.locals())[0].cpu()
+    for a, b in ((0, 5), (5, 6), (6, 27)):
+        part = W.kernel.element_matrices(ADJOINT, STATE, W.locals(), elements=(a, b))[0].cpu()
+        assert torch.equal(part, full[a:b]), (a, b)
     monkeypatch.setenv("HM_CONTRACTION", "mfma")
     for i, j in blocks:
         E = W.kernel.element_matrices(i, j, W.locals())[0].cpu().numpy()
