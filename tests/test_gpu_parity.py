@@ -32,7 +32,8 @@ def pair(n, **kw):
     W = Workload(n, state=state, **kw)
     P = O.Problem(n, order=kw.get("order", 2), perturb=kw.get("perturb", 0.0),
                   numbering=kw.get("numbering", "lattice"), state=state,
-                  density="nonlinear" if kw.get("nonlinear") else "linear")
+                  density="nonlinear" if kw.get("nonlinear") else "linear",
+                  n1d=(kw.get("quadrature_degree") or 4) // 2 + 1)
     return W, P
 
 
@@ -236,14 +237,16 @@ def test_non_finite_raises(gpu):
         prob.jacobian(x)
 
 
+@pytest.mark.parametrize("degree", [4, 6])
 @pytest.mark.parametrize("perturb", [0.0, 0.2])
 @pytest.mark.parametrize("nonlinear", [False, True])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_pair_pass_equals_single_blocks(gpu, precision, nonlinear, perturb):
+def test_pair_pass_equals_single_blocks(gpu, precision, nonlinear, perturb, degree):
     """Two blocks of one shape from one pass over the elements have the bits of the two
     single-block launches, for every pair of blocks that shares a pass; pairs that do not
-    (other shapes, a block that is zero for the density, BF16) are refused and write nothing."""
-    W, P = pair(3, perturb=perturb, nonlinear=nonlinear)
+    (other shapes, a block that is zero for the density, BF16) are refused and write nothing.
+    On the 3- and the 4-point rule."""
+    W, P = pair(3, perturb=perturb, nonlinear=nonlinear, quadrature_degree=degree)
     loc = W.locals()
     K = W.kernel
     single = {(i, j): K.element_matrices(i, j, loc, precision=precision)[0].clone()
