@@ -52,11 +52,12 @@ def test_four_point_rule():
 
 
 # ------------------------------------------------------------------------ basis
+@pytest.mark.parametrize("n1d", [3, 4])
 @pytest.mark.parametrize("order", [1, 2])
 @pytest.mark.parametrize("local", ["lattice", "mfem"])
-def test_basis_invariants(order, local):
+def test_basis_invariants(order, local, n1d):
     nodes = O.lattice_local_nodes(order) if local == "lattice" else O.mfem_local_nodes(order)
-    pts, _ = O.integration_rule(3)
+    pts, _ = O.integration_rule(n1d)
     N, G = O.basis_tables(order, pts, nodes)
     assert np.allclose(N.sum(axis=1), 1.0, atol=1e-14)
     assert np.allclose(G.sum(axis=1), 0.0, atol=1e-13)
@@ -99,17 +100,18 @@ def test_geometry_perturbed_volume():
 
 
 # ------------------------------------------------------------------ element arrays
-def _data(n=2, perturb=0.0, numbering="lattice"):
+def _data(n=2, perturb=0.0, numbering="lattice", n1d=3):
     mesh = O.BoxMesh(n, perturb=perturb)
     Vu = O.H1Space(mesh, 2, numbering)
     Vm = O.H1Space(mesh, 1, numbering)
-    return O.ElementData(mesh, Vu, Vm), Vu, Vm
+    return O.ElementData(mesh, Vu, Vm, n1d), Vu, Vm
 
 
-def test_constant_coefficient_is_textbook_q2():
+@pytest.mark.parametrize("n1d", [3, 4])
+def test_constant_coefficient_is_textbook_q2(n1d):
     """kappa = 1 on a cube of side h: A_e = h (K1 x M1 x M1 + M1 x K1 x M1 + M1 x M1 x K1)
-    with the 1-D P2 matrices on [0, 1]."""
-    data, Vu, Vm = _data(3)
+    with the 1-D P2 matrices on [0, 1]; both rules integrate these products exactly."""
+    data, Vu, Vm = _data(3, n1d=n1d)
     M1 = np.array([[4, 2, -1], [2, 16, 2], [-1, 2, 4]]) / 30.0
     K1 = np.array([[7, -8, 1], [-8, 16, -8], [1, -8, 7]]) / 3.0
     Kref = (np.kron(M1, np.kron(M1, K1)) + np.kron(M1, np.kron(K1, M1))
@@ -158,9 +160,10 @@ def _functional(data, density, u, m, p):
     return float(np.sum(data.wdet * r))
 
 
+@pytest.mark.parametrize("n1d", [3, 4])
 @pytest.mark.parametrize("kind", ["linear", "nonlinear"])
-def test_vectors_are_gradient_of_functional(kind):
-    data, Vu, Vm = _data(2, perturb=0.2)
+def test_vectors_are_gradient_of_functional(kind, n1d):
+    data, Vu, Vm = _data(2, perturb=0.2, n1d=n1d)
     dens = O.Density(kind)
     rng = np.random.default_rng(1)
     u, m, p = rng.standard_normal(Vu.ndof) * 0.5, rng.standard_normal(Vm.ndof) * 0.3, \
@@ -178,10 +181,11 @@ def test_vectors_are_gradient_of_functional(kind):
         assert abs(fd - g @ d) <= 1e-8 * max(1.0, abs(g @ d)), (i, fd, g @ d)
 
 
+@pytest.mark.parametrize("n1d", [3, 4])
 @pytest.mark.parametrize("kind", ["linear", "nonlinear"])
-def test_blocks_are_derivatives_of_vectors(kind):
+def test_blocks_are_derivatives_of_vectors(kind, n1d):
     """Block (i, j) = d(element vectors of slot i) / d(slot j), by central differences."""
-    data, Vu, Vm = _data(2, perturb=0.2)
+    data, Vu, Vm = _data(2, perturb=0.2, n1d=n1d)
     dens = O.Density(kind)
     rng = np.random.default_rng(2)
     u, m, p = rng.standard_normal(Vu.ndof) * 0.5, rng.standard_normal(Vm.ndof) * 0.3, \
