@@ -435,8 +435,10 @@ def _contract(wdet, parts, Nt, Gt, Nc, Gc):
 
 def element_matrices(data, density, i, j, u, m, p, elems=slice(None)):
     """Block ``(i, j)`` element matrices ``(ne, nd_i, nd_j)``: the quadrature route
-    (``_hess_slot_quadrature_fn``, ``kernel.py:1431-1485``) with closed-form pointwise
-    blocks (Appendix A of the path's survey)."""
+    (``_hess_slot_quadrature_fn``, ``kernel.py:1431-1485``) with the pointwise blocks in closed
+    form: the second derivatives of ``r = exp(m) g(u) grad u . grad p - f p`` by slot pair, the
+    ones the reference's ``_point_blocks`` forms by automatic differentiation
+    (``kernel.py:1581-1646``)."""
     F = data.fields(u, m, p, elems)
     Jinv = data.Jinv[elems]
     Nt, Gt, _ = data.tables(i)
