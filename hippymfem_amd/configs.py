@@ -18,7 +18,9 @@
   flop rate and matrix-core busy share against the peaks.
 
 ``python -m hippymfem_amd.configs precision --n 128`` / ``nonlinear --n 96`` /
-``linearization --n 64 [--nonlinear]`` / ``contraction --n 64``.
+``linearization --n 64 [--nonlinear]`` / ``contraction --n 64``.  ``--degree 6`` runs
+``precision`` and ``linearization`` on the reference's default rule for P2 fields (4 x 4 x 4
+Gauss points, ``elementbatch.py:148-156``) instead of the 3 x 3 x 3 rule.
 """
 
 import argparse
@@ -87,8 +89,8 @@ def _blocks(prob, loc, blocks, precisions, steps, n):
     return out
 
 
-def precision_form(n=128, perturb=0.2, steps=3):
-    W = Workload(n, perturb=perturb)
+def precision_form(n=128, perturb=0.2, steps=3, degree=None):
+    W = Workload(n, perturb=perturb, quadrature_degree=degree)
     prob = PDEVariationalProblem(W.spaces, W.density, W.batches, W.bc)
     loc = [torch.as_tensor(v).cuda() for v in W.locals()]
     blocks = [(ADJOINT, STATE), (ADJOINT, PARAMETER), (STATE, PARAMETER), (PARAMETER, PARAMETER)]
@@ -107,8 +109,9 @@ def nonlinear(n=96, steps=3):
     return _blocks(prob, loc, blocks, ("fp32",), steps, n)
 
 
-def linearization(n=64, nonlinear=False, steps=3):
-    W = Workload(n, perturb=0.2, nonlinear=nonlinear, state="fourier" if nonlinear else "uniform")
+def linearization(n=64, nonlinear=False, steps=3, degree=None):
+    W = Workload(n, perturb=0.2, nonlinear=nonlinear, state="fourier" if nonlinear else "uniform",
+                 quadrature_degree=degree)
     prob = PDEVariationalProblem(W.spaces, W.density, W.batches, W.bc,
                                  is_fwd_linear=not nonlinear)
     loc = [torch.as_tensor(v).cuda() for v in W.locals()]
@@ -121,7 +124,7 @@ def linearization(n=64, nonlinear=False, steps=3):
     sep = [b.data.clone() for b in blks]
     t_shr = _time(lambda: prob.blocks(need, x), steps)
     same = all(torch.equal(b.data, d) for b, d in zip(blks, sep))
-    line = {"n": n, "density": "nonlinear" if nonlinear else "linear",
+    line = {"n": n, "density": "nonlinear" if nonlinear else "linear", "degree": W.batches.groups[0].degree,
             "blocks": [NAMES[ij] for ij in need],
             "nnz": int(sum(b.pattern.nnz for b in blks)),
             "separate_ms": t_sep, "shared_ms": t_shr, "speedup": t_sep / t_shr,
@@ -166,12 +169,14 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--nonlinear", action="store_true")
     ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--degree", type=int, default=None,
+                    help="quadrature degree of precision / linearization (default 4: 3x3x3 points)")
     a = ap.parse_args(argv)
     t0 = time.perf_counter()
     if a.which == "precision":
-        precision_form(a.n or 128, steps=a.steps)
+        precision_form(a.n or 128, steps=a.steps, degree=a.degree)
     elif a.which == "linearization":
-        linearization(a.n or 64, nonlinear=a.nonlinear, steps=a.steps)
+        linearization(a.n or 64, nonlinear=a.nonlinear, steps=a.steps, degree=a.degree)
     elif a.which == "contraction":
         contraction(a.n or 64, steps=a.steps)
     else:
