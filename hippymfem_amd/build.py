@@ -43,7 +43,7 @@ def build(verbose=False):
         obj = os.path.join(obj_dir, name + ".o")
         cmd = [cc, "-c", "-x", "hip", "--offload-arch=" + ARCH, *common,
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-               "-munsafe-fp-atomics", src, "-o", obj]
+               src, "-o", obj]
         _run(cmd, verbose)
         objs.append(obj)
     for name in CXX_SOURCES:

@@ -8,8 +8,9 @@
 //
 // The parameter field is the benchmarks' one: N(0, 1) noise on the (n+1)^3 vertex lattice,
 // three passes of the 7-point average over the neighbours that exist, then shifted and scaled
-// to mean 0 and standard deviation 1 (the unbiased one).  The mean and deviation are sums in
-// a fixed order (one block, a fixed tree), so the field is the same bits at every draw.
+// to mean 0 and population standard deviation 1 (divisor n, as numpy's std).  The mean and
+// deviation are sums in a fixed order (one block, a fixed tree), so the field is the same bits
+// at every draw.
 #include <hip/hip_runtime.h>
 
 #include "hm_common.h"
@@ -82,14 +83,14 @@ __device__ double block_sum(const double* __restrict__ v, int64_t n, double shif
   return total;
 }
 
-// stats[0] = mean, stats[1] = unbiased standard deviation
+// stats[0] = mean, stats[1] = population standard deviation (divisor n, numpy's np.std)
 __global__ void __launch_bounds__(FS_THREADS) k_field_stats(const double* __restrict__ f, int64_t n, double* stats) {
   __shared__ double red[FS_THREADS];
   const double mean = block_sum(f, n, 0.0, false, red) / (double)n;
   const double ss = block_sum(f, n, mean, true, red);
   if (threadIdx.x == 0) {
     stats[0] = mean;
-    stats[1] = n > 1 ? sqrt(ss / (double)(n - 1)) : 1.0;
+    stats[1] = n > 0 && ss > 0.0 ? sqrt(ss / (double)n) : 1.0;
   }
 }
 

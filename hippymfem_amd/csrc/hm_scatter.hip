@@ -1,5 +1,5 @@
 // The scatter of element arrays into CSR values and dof vectors: a deterministic
-// segmented reduction with no float atomics.
+// segmented reduction with no atomics on values (in memory or in LDS).
 //
 // Every CSR row owns its sum.  Its entries come from its element rows (runs t = e*nd_t + a,
 // ascending, from the pattern build), each a contiguous run of nd_c entries of E, and the
@@ -18,6 +18,9 @@
 #include "hm_device.h"
 
 namespace hm {
+
+// LDS of the row buffers of one block (the dynamic LDS a launch takes without asking for more)
+constexpr size_t SCATTER_LDS = 64 * 1024;
 
 template <typename CodeT>
 struct Drop;
@@ -117,12 +120,13 @@ __global__ void __launch_bounds__(256) k_scatter_rows(hm_scatter_plan S, const E
           v[k] = E[(t - t0) * nd_c + lane];
         }
       }
-      // one LDS add a run, run after run: a wavefront's LDS operations complete in issue
-      // order and the entries of one run have distinct places, so every sum takes its entries
-      // in ascending run order
+      // one read-add-write a run, run after run: the entries of one run have distinct places
+      // (no lane of the row reads a place another lane writes in the same run) and a
+      // wavefront's LDS operations complete in issue order, so every sum takes its entries in
+      // ascending run order; no atomics
 #pragma unroll
       for (int k = 0; k < BATCH; ++k)
-        if (c[k] != DROP) atomicAdd(&buf[c[k]], (double)v[k]);
+        if (c[k] != DROP) buf[c[k]] = buf[c[k]] + (double)v[k];
       __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
       __builtin_amdgcn_wave_barrier();
     }
@@ -239,8 +243,17 @@ static int scatter_matrix(const hm_scatter_plan* S, const void* E, int e_dtype, 
   HM_REQUIRE(e0 >= 0 && e0 <= e1 && e1 <= S->ne, "elements [%lld, %lld) outside [0, %lld)",
              (long long)e0, (long long)e1, (long long)S->ne);
   HM_REQUIRE(S->code_bytes == 1 || S->code_bytes == 2, "code width %d is neither 1 nor 2", S->code_bytes);
-  HM_REQUIRE(S->max_row >= 0 && (size_t)S->max_row * 8 * 8 <= 64 * 1024,
-             "a row of %d entries does not fit the row buffers", S->max_row);
+  HM_REQUIRE(S->max_row >= 0, "negative longest row %d", S->max_row);
+  {
+    // a block holds 256 / LPR row buffers of max_row fp64 values, LPR as launch_rows picks it
+    const int lpr = S->nd_c <= 8 ? 8 : S->nd_c <= 16 ? 16 : S->nd_c <= 32 ? 32 : 64;
+    const size_t need = (size_t)(256 / lpr) * (size_t)S->max_row * sizeof(double);
+    if (need > SCATTER_LDS)
+      HM_FAIL(HM_ERR_UNSUPPORTED,
+              "a row of %d entries needs %zu bytes of row buffers for %d rows a block; the scatter "
+              "takes at most %zu",
+              S->max_row, need, 256 / lpr, (size_t)SCATTER_LDS);
+  }
   if (S->nrow == 0) return HM_OK;
   const int64_t t0 = e0 * S->nd_t, t1 = e1 * S->nd_t;
   hipStream_t st = (hipStream_t)stream;

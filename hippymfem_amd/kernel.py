@@ -84,6 +84,17 @@ class QuadratureKernel:
         self._keep = (u, m, p, g, tu, tm)
         return P, (u, m, p)
 
+    @staticmethod
+    def _hold(tensors, stream):
+        """Keep the temporaries a launch on ``stream`` reads alive until that stream has used
+        them: the caching allocator orders a free against the current stream only, so a
+        launch on another stream records its use (``Tensor.record_stream``)."""
+        if stream is None or stream == torch.cuda.current_stream():
+            return
+        for t in tensors:
+            if isinstance(t, torch.Tensor) and t.is_cuda:
+                t.record_stream(stream)
+
     # ------------------------------------------------------------ element arrays
     def element_matrices(self, i, j, local_arrays, params=(), consistent=None,
                          precision="fp64", elements=None, out=None, stream=None, nonfinite=None):
@@ -107,6 +118,7 @@ class QuadratureKernel:
         _lib.check(_lib.lib().hm_element_matrices(
             _lib.ctypes.byref(P), int(i), int(j), int(a), int(b), code, _lib.ptr(out),
             _lib.stream_handle(stream)), "element matrices (%d, %d)" % (i, j))
+        self._hold(keep, stream)
         return [out]
 
     def consistent_block(self, i, j, local_arrays, symmetric=False, precision="fp32",
@@ -134,6 +146,7 @@ class QuadratureKernel:
         _lib.check(_lib.lib().hm_consistent_block(
             _lib.ptr(K), _lib.ptr(Rj), _lib.ptr(Ri), b - a, self.nd[i], self.nd[j],
             1 if symmetric else 0, _lib.ptr(out), _lib.stream_handle(stream)), "consistent block")
+        self._hold((K, Rj, Ri), stream)
         self._keep = (self._keep, K, Rj, Ri)
         return out
 
@@ -161,6 +174,7 @@ class QuadratureKernel:
         if rc == _lib.HM_ERR_UNSUPPORTED:
             return None
         _lib.check(rc, "element matrices (%d, %d) and (%d, %d)" % (i1, j1, i2, j2))
+        self._hold(keep, stream)
         return o1, o2
 
     def element_matrices_many(self, pairs, local_arrays, precision="fp64"):
@@ -217,6 +231,7 @@ class QuadratureKernel:
         _lib.check(_lib.lib().hm_element_vectors(_lib.ctypes.byref(P), int(i), int(a), int(b),
                                                  _lib.ptr(out), _lib.stream_handle(stream)),
                    "element vectors (%d)" % i)
+        self._hold(keep, stream)
         return [out]
 
     def element_hvp(self, i, j, local_arrays, jdir_local, params=(), elements=None, out=None,
@@ -232,6 +247,7 @@ class QuadratureKernel:
                                              int(a), int(b), _lib.ptr(out),
                                              _lib.stream_handle(stream)),
                    "element products (%d, %d)" % (i, j))
+        self._hold(keep + (d,), stream)
         return [out]
 
 

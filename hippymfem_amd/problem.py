@@ -90,14 +90,30 @@ class BlockAssembly:
         e_dtype = torch.float64 if precision == "fp64" else torch.float32
         if value_dtype is None:
             value_dtype = e_dtype
-        ne = kernel.group.ne
-        self.E = torch.empty((ne, kernel.nd[i], kernel.nd[j]), dtype=e_dtype, device=dev)
+        self.e_dtype = e_dtype
+        self._E = None
         self.data = torch.empty(self.pattern.nnz, dtype=value_dtype, device=dev)
         self.row_defect = None
         if row_sums and Vi is Vj and value_dtype != torch.float64:
             self.row_defect = torch.empty(self.pattern.nrow, dtype=torch.float64, device=dev)
         # set by the element kernel when it writes a value that is not finite
         self.nonfinite = torch.zeros(1, dtype=torch.int32, device=dev)
+
+    @property
+    def E(self):
+        """The resident element matrices ``(ne, nd_i, nd_j)``, made at the first use."""
+        if self._E is None:
+            k = self.kernel
+            self._E = torch.empty((k.group.ne, k.nd[self.i], k.nd[self.j]), dtype=self.e_dtype,
+                                  device=self.data.device)
+        return self._E
+
+    def fresh_values(self):
+        """An uninitialized values array (and row defect) of this block's shape, for a result
+        that owns its values."""
+        d = torch.empty_like(self.data)
+        rd = None if self.row_defect is None else torch.empty_like(self.row_defect)
+        return d, rd
 
     def clear_flag(self, stream=None):
         with torch.cuda.stream(stream) if stream is not None else contextlib.nullcontext():
@@ -112,19 +128,29 @@ class BlockAssembly:
         """Whether the last element pass wrote finite values only (one int32 read back)."""
         return not bool(self.nonfinite.item())
 
-    def scatter(self, stream=None):
-        return self.pattern.data([self.E], plan=self.dplan, out=self.data, stream=stream,
-                                 row_defect=self.row_defect)
+    def scatter(self, stream=None, out=None, row_defect=None):
+        """The resident element matrices into CSR values: into the resident ``data`` (and
+        ``row_defect``), or into ``out`` (and ``row_defect``) when given."""
+        if out is None:
+            out, row_defect = self.data, self.row_defect
+        return self.pattern.data([self.E], plan=self.dplan, out=out, stream=stream,
+                                 row_defect=row_defect)
 
-    def finish(self):
-        A = finish_block(self.plan, self.data)
-        A.row_defect = self.row_defect
+    def finish(self, data=None, row_defect=None):
+        """The matrix over ``data`` (the resident values by default)."""
+        if data is None:
+            data, row_defect = self.data, self.row_defect
+        A = finish_block(self.plan, data)
+        A.row_defect = row_defect
         return A
 
-    def assemble(self, local_arrays, stream=None):
+    def assemble(self, local_arrays, stream=None, out=None, row_defect=None):
+        """Element kernel, then scatter.  The matrix aliases the resident values unless
+        ``out`` (and ``row_defect``) is given: a repeated assembly into resident memory
+        allocates nothing, and the next assembly of this block overwrites it."""
         self.element_matrices(local_arrays, stream)
-        self.scatter(stream)
-        return self.finish()
+        self.scatter(stream, out=out, row_defect=row_defect)
+        return self.finish(out, row_defect) if out is not None else self.finish()
 
 
 class PDEVariationalProblem:
@@ -167,9 +193,13 @@ class PDEVariationalProblem:
     def block(self, i, j, x, precision="fp64"):
         """Block ``(i, j)`` at ``x`` with its elimination (``_block``/``_domain_block``,
         ``PDEVariationalProblem.py:448-525``); ``RuntimeError`` when an element matrix is
-        not finite (the check of ``_domain_block``, ``:495-525``)."""
+        not finite (the check of ``_domain_block``, ``:495-525``).  The matrix owns its
+        values: a later assembly at another point leaves it as it is, as the reference's
+        matrices are (a line search or a finite-difference check keeps the linearization
+        point's blocks, ``PDEVariationalProblem.py:562-568``)."""
         blk = self.block_assembly(i, j, precision)
-        A = blk.assemble(self._locals(x))
+        out, rd = blk.fresh_values()
+        A = blk.assemble(self._locals(x), out=out, row_defect=rd)
         if not blk.finite():
             require_finite_arrays([blk.E], "block (%d, %d)" % (i, j))
         return A
@@ -182,6 +212,7 @@ class PDEVariationalProblem:
         loc = self._locals(x)
         todo = list(dict.fromkeys(tuple(ij) for ij in pairs))
         passes = []
+        results = {}
         while todo:
             ij = todo.pop(0)
             blk = self.block_assembly(ij[0], ij[1], precision)
@@ -197,7 +228,10 @@ class PDEVariationalProblem:
             else:
                 blk.element_matrices(loc)
             for d in done:
-                self.block_assembly(d[0], d[1], precision).scatter()
+                b = self.block_assembly(d[0], d[1], precision)
+                out, rd = b.fresh_values()
+                b.scatter(out=out, row_defect=rd)
+                results[d] = (out, rd)
             passes.append((blk, done))
         # every pass launched before the first read of a flag (one flag a pass, the first
         # block's); a set flag is traced to its block
@@ -206,7 +240,8 @@ class PDEVariationalProblem:
                 for d in done:
                     require_finite_arrays([self.block_assembly(d[0], d[1], precision).E],
                                           "block (%d, %d)" % d)
-        return {d: self.block_assembly(d[0], d[1], precision).finish()
+        # every matrix owns its values (``block``)
+        return {d: self.block_assembly(d[0], d[1], precision).finish(*results[d])
                 for _, done in passes for d in done}
 
     def jacobian(self, x, precision="fp64"):

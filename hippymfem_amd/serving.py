@@ -81,7 +81,8 @@ def _seed_tensor(seed, device):
 def device_field(n, seed, device, passes=3):
     """The seeded parameter field on the ``(n+1)^3`` vertex lattice, drawn on the device
     (``hm_serving_field``): N(0, 1) noise, ``passes`` of the 7-point average over the
-    neighbours that exist, then mean 0 and std 1 (the field of the benchmarks)."""
+    neighbours that exist, then mean 0 and population std 1 (the field of the benchmarks,
+    ``workloads.smoothed_noise``)."""
     n1 = int(n) + 1
     work = torch.empty(2 * n1 ** 3 + 2, dtype=torch.float64, device=device)
     out = torch.empty(n1 ** 3, dtype=torch.float64, device=device)

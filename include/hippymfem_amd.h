@@ -245,7 +245,7 @@ int hm_csr_matvec(int64_t nrow, const int32_t* indptr, const int32_t* indices,
  * hm_serving_normal: out (n) fp64 N(0, 1) values of (*seed, salt).
  * hm_serving_field: the benchmarks' parameter field on the n1^3 vertex lattice (x fastest):
  *   N(0, 1) noise of (*seed, 0), `passes` 7-point averages over the neighbours that exist,
- *   then mean 0 and unbiased standard deviation 1.  work (2 n1^3 + 2) fp64; out (n1^3) fp64
+ *   then mean 0 and population standard deviation 1 (divisor n, as numpy's std).  work (2 n1^3 + 2) fp64; out (n1^3) fp64
  *   in lattice order, or out[lattice_to_dof[i]] when lattice_to_dof is not NULL. */
 int hm_serving_normal(const int64_t* seed, int64_t salt, int64_t n, double* out, void* stream);
 int hm_serving_field(const int64_t* seed, int n1, int passes, double* work,

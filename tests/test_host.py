@@ -168,3 +168,23 @@ def test_product_fails_loudly_without_gpu():
     W = Workload(2)
     with pytest.raises(RuntimeError):
         W.kernel.element_matrices(2, 0, W.locals())
+
+
+@needs_lib
+@pytest.mark.parametrize("nd_c,longest,lanes", [(8, 257, 8), (8, 300, 8), (16, 520, 16),
+                                                (27, 1100, 32)])
+def test_scatter_refuses_rows_beyond_its_buffers(nd_c, longest, lanes):
+    """The scatter holds 256 / LPR row buffers of the longest row in LDS, LPR the least
+    power of two of at least nd_c lanes; a longer row is refused before any launch with
+    HM_ERR_UNSUPPORTED (a Q1-column row of more than 256 entries, with 8 lanes a row)."""
+    L = hm_lib.lib()
+    assert (256 // lanes) * longest * 8 > 64 * 1024
+    plan = hm_lib.HmScatterPlan(nrow=4, nnz=4 * longest, ne=1, nd_t=8, nd_c=nd_c,
+                                code_bytes=2, max_row=longest)
+    rc = L.hm_scatter_matrix(ctypes.byref(plan), None, hm_lib.FP64, 0, 1, None, hm_lib.FP64,
+                             0, None)
+    assert rc == hm_lib.HM_ERR_UNSUPPORTED
+    msg = L.hm_last_error().decode()
+    assert "row of %d entries" % longest in msg and str(256 // lanes) in msg
+    with pytest.raises(NotImplementedError):
+        hm_lib.check(rc, "scatter")

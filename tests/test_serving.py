@@ -41,6 +41,20 @@ def test_summary_by_model():
 
 
 @pytest.mark.gpu
+def test_served_field_has_population_std_one(gpu):
+    """The served parameter field is normalized as the benchmarks' field is
+    (``workloads.smoothed_noise``: ``np.std``, divisor n)."""
+    from hippymfem_amd.serving import device_field
+
+    for n in (4, 9):
+        f = device_field(n, 77, gpu).cpu().numpy()
+        assert f.size == (n + 1) ** 3
+        assert abs(f.mean()) < 1e-13
+        assert abs(f.std() - 1.0) < 1e-13, f.std()           # population deviation
+        assert abs(f.std(ddof=1) - 1.0) > 1e-4                # not the unbiased one
+
+
+@pytest.mark.gpu
 def test_served_product_matches_oracle(gpu):
     from oracle import fe_oracle as O
 
