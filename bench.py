@@ -144,13 +144,16 @@ def _parallel_cpu(P, ip, slot, nnz, kill, workers):
 
 def timed_steps(blk, loc, steps, warmup, stream):
     """Warm-up, then ``steps`` assemblies bracketed by synchronizations; HIP events on the
-    stream the kernels run on split every step into its element-kernel and scatter launches."""
+    stream the kernels run on time every launch: the fused kernel (one launch a step) on the
+    fused route, else the element kernel and the scatter.  Returns (wall time a step, first
+    launch, second launch or 0)."""
     import gc
 
     import torch
 
+    fused = blk.fused_plan() is not None
     for _ in range(warmup):
-        blk.assemble(loc, stream=stream)
+        blk.assemble_values(loc, stream=stream)
     torch.cuda.synchronize()
     ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
     # no garbage collection pause inside the timed region (a full collection after building
@@ -161,9 +164,13 @@ def timed_steps(blk, loc, steps, warmup, stream):
         t0 = time.perf_counter()
         for k in range(steps):
             ev[k][0].record(stream)
-            blk.element_matrices(loc, stream=stream)
-            ev[k][1].record(stream)
-            blk.scatter(stream=stream)
+            if fused:
+                blk.assemble_values(loc, stream=stream)
+                ev[k][1].record(stream)
+            else:
+                blk.element_matrices(loc, stream=stream)
+                ev[k][1].record(stream)
+                blk.scatter(stream=stream)
             ev[k][2].record(stream)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
@@ -171,7 +178,25 @@ def timed_steps(blk, loc, steps, warmup, stream):
         gc.enable()
     tk = float(np.mean([ev[k][0].elapsed_time(ev[k][1]) for k in range(steps)])) * 1e-3
     ts = float(np.mean([ev[k][1].elapsed_time(ev[k][2]) for k in range(steps)])) * 1e-3
-    return wall / steps, tk, ts
+    return wall / steps, tk, (0.0 if fused else ts)
+
+
+def fused_bytes(blk, W):
+    """HBM bytes the fused launch moves at the least: the values written (the single-pass
+    floor, B_min = vsize nnz), the plan read (the stencil map, one byte an entry; the row
+    headers, 8 bytes a row) and the element data (vertex and dof ids, vertex coordinates and
+    dof values, the fields only where the block reads them)."""
+    pat = blk.pattern
+    k = blk.kernel
+    vsize = blk.data.element_size()
+    ne = pat.ne
+    Vu, Vm = k.spaces[0], k.spaces[1]
+    fields = not (blk.i == 2 and blk.j == 0 and not k.density.nonlinear)
+    b_min = vsize * pat.nnz
+    b_plan = pat.nnz + 8 * pat.nrow
+    b_inputs = (4 * 8 * ne + 24 * W.mesh.nvert + 4 * ne * Vm.nd + 8 * Vm.ndof +
+                ((4 * ne * Vu.nd + 16 * Vu.ndof) if fields else 0))
+    return b_min, b_min + b_plan + b_inputs
 
 
 def algorithmic_bytes(blk, W):
@@ -207,7 +232,7 @@ def measured_traffic():
     except OSError:
         return {}
     out = {}
-    for key in ("element_kernel", "scatter"):
+    for key in ("element_kernel", "scatter", "fused"):
         m = re.search(r"^%s_traffic_bytes:\s*([0-9.eE+]+)" % key, text, re.M)
         if m:
             out[key] = float(m.group(1))
@@ -223,19 +248,24 @@ def fp32_form(prob, loc, blk64, W, args, stream):
     from hippymfem_amd import ADJOINT, STATE
 
     blk = prob.block_assembly(ADJOINT, STATE, precision="fp32")
+    fused = blk.fused_plan() is not None
     step, tk, ts = timed_steps(blk, loc, args.steps, args.warmup, stream)
     d64 = blk64.data
     d32 = blk.data.double()
     err_max = float((d32 - d64).abs().max() / d64.abs().max())
     err_fro = float(torch.linalg.vector_norm(d32 - d64) / torch.linalg.vector_norm(d64))
-    b_elem, b_scatter = algorithmic_bytes(blk, W)
-    b_e2e = b_elem + b_scatter
-    return {"ms_per_step": 1e3 * step, "value": blk.pattern.nnz / step, "unit": "nnz/s",
-            "element_kernel_ms": 1e3 * tk, "scatter_ms": 1e3 * ts,
-            "speedup_vs_fp64": None,
-            "e2e": {"bytes": b_e2e, "GBs": b_e2e / step / 1e9,
-                    "frac": b_e2e / step / 1e9 / HBM_PEAK_GBS},
-            "max_rel_err_vs_fp64": err_max, "fro_rel_err_vs_fp64": err_fro}
+    if fused:
+        _, b_e2e = fused_bytes(blk, W)
+        split = {"fused_kernel_ms": 1e3 * tk}
+    else:
+        b_elem, b_scatter = algorithmic_bytes(blk, W)
+        b_e2e = b_elem + b_scatter
+        split = {"element_kernel_ms": 1e3 * tk, "scatter_ms": 1e3 * ts}
+    return dict({"ms_per_step": 1e3 * step, "value": blk.pattern.nnz / step, "unit": "nnz/s",
+                 "route": "fused" if fused else "two-pass", "speedup_vs_fp64": None,
+                 "e2e": {"bytes": b_e2e, "GBs": b_e2e / step / 1e9,
+                         "frac": b_e2e / step / 1e9 / HBM_PEAK_GBS},
+                 "max_rel_err_vs_fp64": err_max, "fro_rel_err_vs_fp64": err_fro}, **split)
 
 
 def linearization_point(prob, loc, args):
@@ -283,10 +313,13 @@ def default_rule(args, loc, stream):
     W = Workload(args.n, quadrature_degree=6)
     prob = PDEVariationalProblem(W.spaces, W.density, W.batches, W.bc)
     blk = prob.block_assembly(ADJOINT, STATE)
+    fused = blk.fused_plan() is not None
     step, tk, ts = timed_steps(blk, loc, args.steps, args.warmup, stream)
     out = {"quadrature": "4x4x4 Gauss (degree 6)", "ms_per_step": 1e3 * step,
            "value": blk.pattern.nnz / step, "unit": "nnz/s",
-           "element_kernel_ms": 1e3 * tk, "scatter_ms": 1e3 * ts}
+           "route": "fused" if fused else "two-pass"}
+    out.update({"fused_kernel_ms": 1e3 * tk} if fused else
+               {"element_kernel_ms": 1e3 * tk, "scatter_ms": 1e3 * ts})
     del blk, prob
     torch.cuda.empty_cache()
     return out
@@ -327,28 +360,52 @@ def main():
     blk = prob.block_assembly(ADJOINT, STATE)
     loc = [torch.as_tensor(v).to(dev) for v in (W.u, W.m, W.p)]
     stream = torch.cuda.current_stream()
-    step, t_kernel, t_scatter = timed_steps(blk, loc, args.steps, args.warmup, stream)
+    fused = blk.fused_plan() is not None
+    step, t_kernel, _ = timed_steps(blk, loc, args.steps, args.warmup, stream)
     pat = blk.pattern
-    b_elem, b_scatter = algorithmic_bytes(blk, W)
     traffic = measured_traffic()
     # the contraction's algorithmic flops: the dense formulation E = Psi^T (w D) Psi of the
     # stiffness-type block, 2 * 27 * 27 * 81 an element (the sum-factorized kernel does fewer)
     flops = 2.0 * pat.nd_t * pat.nd_c * 81 * pat.ne
-    kernels = {
-        "k_sumfact": {"ms": 1e3 * t_kernel, "bytes": b_elem,
-                      "GBs": b_elem / t_kernel / 1e9,
-                      "traffic": traffic.get("element_kernel"),
-                      "algorithmic_flops": flops, "TFLOPs": flops / t_kernel / 1e12,
-                      "frac_fp64_peak": flops / t_kernel / 1e12 / FP64_PEAK_TFLOPS},
-        "k_scatter_rows": {"ms": 1e3 * t_scatter, "bytes": b_scatter,
-                           "GBs": b_scatter / t_scatter / 1e9,
-                           "traffic": traffic.get("scatter")},
-    }
-    dom = max(kernels, key=lambda k: kernels[k]["ms"])
-    d = kernels[dom]
-    roof = {"kernel": dom, "bound": "hbm", "achieved": d["GBs"], "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": d["GBs"] / HBM_PEAK_GBS, "traffic": d["traffic"]}
-    b_e2e = b_elem + b_scatter
+    # the two-pass route of the same block (element matrices in memory, then the scatter)
+    blk2 = prob.block_assembly(ADJOINT, STATE, route="two-pass")
+    step2, t2_kernel, t2_scatter = timed_steps(blk2, loc, args.steps, args.warmup, stream)
+    b_elem, b_scatter = algorithmic_bytes(blk2, W)
+    two_pass = {
+        "ms_per_step": 1e3 * step2, "value": pat.nnz / step2, "unit": "nnz/s",
+        "kernels": {
+            "k_sumfact": {"ms": 1e3 * t2_kernel, "bytes": b_elem,
+                          "GBs": b_elem / t2_kernel / 1e9,
+                          "traffic": traffic.get("element_kernel"),
+                          "algorithmic_flops": flops, "TFLOPs": flops / t2_kernel / 1e12},
+            "k_scatter_rows": {"ms": 1e3 * t2_scatter, "bytes": b_scatter,
+                               "GBs": b_scatter / t2_scatter / 1e9,
+                               "traffic": traffic.get("scatter")}},
+        "e2e": {"bytes": b_elem + b_scatter,
+                "frac": (b_elem + b_scatter) / step2 / 1e9 / HBM_PEAK_GBS},
+        "same_bits_as_fused": bool(torch.equal(blk.data, blk2.data)) if fused else None}
+    if fused:
+        b_min, b_alg = fused_bytes(blk, W)
+        kernels = {"k_fused": {"ms": 1e3 * t_kernel, "bytes_min": b_min, "bytes": b_alg,
+                               "GBs_min": b_min / t_kernel / 1e9, "GBs": b_alg / t_kernel / 1e9,
+                               "traffic": traffic.get("fused"),
+                               "algorithmic_flops": flops, "TFLOPs": flops / t_kernel / 1e12,
+                               "frac_fp64_peak": flops / t_kernel / 1e12 / FP64_PEAK_TFLOPS,
+                               "tile": list(blk.fused_plan().tile)}}
+        d = kernels["k_fused"]
+        roof = {"kernel": "k_fused", "bound": "hbm", "achieved": d["GBs_min"],
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": d["GBs_min"] / HBM_PEAK_GBS,
+                "traffic": d["traffic"],
+                "basis": "single-pass floor B_min = 8 nnz (the values written); the launch is "
+                         "bound by its FP64 arithmetic, see kernels.k_fused.TFLOPs"}
+        b_e2e = b_alg
+    else:
+        kernels = two_pass["kernels"]
+        dom = max(kernels, key=lambda k: kernels[k]["ms"])
+        d = kernels[dom]
+        roof = {"kernel": dom, "bound": "hbm", "achieved": d["GBs"], "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": d["GBs"] / HBM_PEAK_GBS, "traffic": d["traffic"]}
+        b_e2e = b_elem + b_scatter
     out = {
         "metric": METRIC, "value": pat.nnz / step, "unit": "nnz/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * step,
@@ -357,11 +414,12 @@ def main():
         "config": {"workload": "%d^3 hex P2 Jacobian A, exp(m) diffusion, FP64, 3x3x3 Gauss, "
                                "Dirichlet x=0,1 folded" % args.n,
                    "elements": pat.ne, "nnz": pat.nnz, "entries": pat.ne * pat.nd_t * pat.nd_c,
-                   "parallelism": "1 GPU"},
+                   "route": "fused" if fused else "two-pass", "parallelism": "1 GPU"},
         "roofline": roof,
         "kernels": kernels,
         "e2e": {"bytes": b_e2e, "GBs": b_e2e / step / 1e9,
                 "frac": b_e2e / step / 1e9 / HBM_PEAK_GBS},
+        "two_pass": two_pass,
     }
     out["linearization_point"] = linearization_point(prob, loc, args)
     if not args.no_default_rule:

@@ -25,8 +25,8 @@ EXPORTS = ("hm_last_error", "hm_version", "hm_element_matrices", "hm_element_mat
            "hm_consistent_block", "hm_element_vectors", "hm_element_hvp", "hm_pattern_build",
            "hm_pattern_place", "hm_pattern_repeats", "hm_pattern_work_bytes",
            "hm_pattern_build_device", "hm_pattern_place_device", "hm_scatter_matrix",
-           "hm_scatter_matrix_defect", "hm_scatter_vector", "hm_csr_matvec", "hm_serving_normal",
-           "hm_serving_field")
+           "hm_scatter_matrix_defect", "hm_scatter_vector", "hm_csr_matvec", "hm_fused_tile",
+           "hm_assemble_fused", "hm_serving_normal", "hm_serving_field")
 
 P = ctypes.c_void_p
 I32, I64, DBL = ctypes.c_int32, ctypes.c_int64, ctypes.c_double
@@ -46,6 +46,13 @@ class HmScatterPlan(ctypes.Structure):
     _fields_ = [("nrow", I64), ("nnz", I64), ("ne", I64), ("nd_t", I32), ("nd_c", I32),
                 ("code_bytes", I32), ("max_row", I32), ("indptr", P), ("rptr", P),
                 ("runs", P), ("codes", P), ("rowmode", P), ("diag", DBL), ("rowhead", P)]
+
+
+class HmFusedPlan(ctypes.Structure):
+    """``hm_fused_plan`` of ``include/hippymfem_amd.h``."""
+    _fields_ = [("nx", I32), ("ny", I32), ("nz", I32), ("bx", I32), ("by", I32),
+                ("reserved", I32), ("hdr", P), ("hdr_off", P), ("sigma", P), ("sig_off", P),
+                ("diag", DBL)]
 
 
 #: int32 words of a row header (``HM_ROWHEAD``)
@@ -94,6 +101,9 @@ def lib():
         "hm_scatter_vector": (ctypes.c_int, [I64, ctypes.c_int, P, P, P, I64, I64, P, P,
                                              ctypes.c_int, P]),
         "hm_csr_matvec": (ctypes.c_int, [I64, P, P, P, ctypes.c_int, P, P, P]),
+        "hm_fused_tile": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P]),
+        "hm_assemble_fused": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, P, ctypes.c_int, P,
+                                             ctypes.c_int, P]),
         "hm_serving_normal": (ctypes.c_int, [P, I64, I64, P, P]),
         "hm_serving_field": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, P, P, P, P]),
     }

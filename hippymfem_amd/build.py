@@ -16,7 +16,7 @@ OUT_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT_DIR, "libhippymfem_amd.so")
 ARCH = os.environ.get("HM_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["hm_element.hip", "hm_mfma.hip", "hm_sumfact.hip", "hm_scatter.hip",
+HIP_SOURCES = ["hm_element.hip", "hm_mfma.hip", "hm_sumfact.hip", "hm_fused.hip", "hm_scatter.hip",
                "hm_pattern_device.hip", "hm_field.hip"]
 CXX_SOURCES = ["hm_abi.cpp", "hm_pattern.cpp"]
 

@@ -1,0 +1,843 @@
+// Fused assembly of a block on a box mesh: the element kernel and the scatter into CSR values
+// in one kernel, so the element matrices never go to memory.
+//
+// Owner computes.  The n_x x n_y x n_z hexahedra (element e = ex + n_x (ey + n_y ez)) are cut into
+// columns of bx x by elements (tiles), each swept from bottom to top by one block.  A tile owns the
+// CSR rows of the test-lattice points (p_t the test order) with x in [p_t x0, p_t x1) (the last tile
+// also x = p_t n_x), likewise y, every z.  Each layer of the sweep takes the tile's elements and a
+// one-element halo on its lower x and y sides (the elements whose upper faces hold owned rows):
+// (bx + 1)(by + 1) elements a layer.  A block of threads
+//   1. puts the layer's element data (vertex coordinates, dof values) in LDS (loaded a layer ahead),
+//   2. acts as the (element, point) pairs of the layer: geometry, fields and the block's pointwise
+//      pairs into LDS (the same code as the element-matrix kernel, hm_sumfact.hip),
+//   3. acts as the contraction tasks (element, a1, b1, a2) of the owned rows only: stages 1-3 of the
+//      sum factorization in registers (b2, a3, b3 in registers), the same operations in the same
+//      order as the element-matrix kernel, so every entry has its bits,
+//   4. adds the entries into LDS row buffers of the lattice plane the rows lie in, in sub-steps
+//      that order the adds of every CSR value by ascending element: a3 = 0, 1, 2 (the plane of
+//      z = p_t ez holds the previous layer's a3 = p_t adds already), and within one a3 the rows of
+//      local y >= 1 before y = 0 and of local x >= 1 before x = 0.  Two elements of one sub-step
+//      never share a row (rows of one parity in x come from one local x), so the adds of a sub-step
+//      are plain read-add-writes without atomics, and the sub-steps are ordered by block barriers.
+//      Every CSR value is so the sum of its entries in ascending element order, from 0: the order
+//      of np.bincount (hippymfem/fem/pattern.py:788), and the bits of the two-pass scatter of the
+//      same element matrices,
+//   5. writes a plane out once no later layer adds to it (after a3 = 0 and a3 = 1; the plane of
+//      a3 = p_t is carried into the next layer) as whole CSR rows, and clears its buffer.
+// A row buffer holds the row's columns as a dense stencil (the trial-lattice box the row couples
+// to), so an entry's place follows from its local indices; a per-entry map of the plan (sigma, in
+// tile order, one byte an entry) gives the stencil place of every CSR position, with the dropped
+// columns of the elimination flagged.  Tiles may be cut into chunks of layers, each recomputing
+// the layer below it (its a3 = p_t rows only) as a halo, to balance the blocks over the device.
+// The element matrices, the pointwise pairs and the stencil rows never leave the compute unit.
+#include <hip/hip_runtime.h>
+
+#include "hm_common.h"
+#include "hm_device.h"
+#include "hm_point.h"
+#include "hm_sumfact.h"
+
+namespace hm {
+
+constexpr int FU_MAXL = 17;     // lattice points of a tile a direction (p_t bx + 1)
+constexpr int FU_MAXM = 64;     // element slots of a layer
+constexpr int FU_MAXT = 1024;   // contraction tasks of a layer
+constexpr int FU_ITEMS = 2;     // element-data loads a thread a layer
+constexpr int FU_SIGV = 2;      // 16-byte stencil-map loads a thread a layer
+constexpr int FU_SLOTS = 2;     // planes of staged tables
+constexpr int FU_MAXZ = 128;    // layers of a chunk
+constexpr int FU_MAXS2D = 2048;  // stencil places of a tile's lattice plane a unit of height
+constexpr int FU_EB = 8;         // plane entries a thread writes out in one batch
+
+// The box, the tile grid and the plan arrays (kernel argument).
+struct FusedGeom {
+  int nx, ny, nz;
+  int bx, by, ntx, nty;
+  int lz_chunk, nchunk;
+  const int2* hdr;              // every row's {CSR offset, row mode}, in tile order
+  const int64_t* hdr_off;       // (nplane, tiles): first row of each (plane, tile)
+  const uint8_t* sigma;         // stencil place of every CSR entry, tile order; bit 7: dropped
+  const int64_t* sig_off;       // (nplane, tiles): 16-byte aligned start of each (plane, tile)
+  double diag;
+  // the LDS slot of every pointwise pair (pairs of equal values share one: the symmetric
+  // reference-gradient pairs (k, l), (l, k), and value/gradient pairs of a block of one slot
+  // with itself), the slots, and the pairs that write their slot
+  int ws[SF_MAXPAIR];
+  int nslot;
+  uint32_t wfirst;
+};
+
+// Sizes of a tile's LDS, for the launch and the host's choice of tile.
+struct FusedSizes {
+  int m_max, t_max, rows_max, sx_max, sy_max, dw, ew;
+  size_t table_words, elem_words, plane0_words, plane1_words, sig_bytes, bytes;
+};
+
+// The stencil width a direction of a lattice point l of order pt (n elements), trial order pc.
+__host__ __device__ inline int fu_width(int l, int pt, int n, int pc) {
+  const bool shared = (l % pt == 0) && l > 0 && l < pt * n;
+  return pc * (shared ? 2 : 1) + 1;
+}
+
+template <int NT, int NC, int NQ>
+__host__ inline FusedSizes fu_sizes(int bx, int by, int nslot, int nd_u, int nd_m, bool fields, int nz_max) {
+  constexpr int pt = NT - 1, pc = NC - 1, NQ3 = NQ * NQ * NQ;
+  FusedSizes z{};
+  z.m_max = (bx + 1) * (by + 1);
+  z.t_max = NC * (bx * by * NT * NT + (bx + by) * NT + 1);
+  z.rows_max = (pt * bx + 1) * (pt * by + 1);
+  z.sx_max = bx * (2 * pc + 1) + (pt - 1) * bx * (pc + 1) + (pc + 1);
+  z.sy_max = by * (2 * pc + 1) + (pt - 1) * by * (pc + 1) + (pc + 1);
+  z.dw = (24 + nd_m + (fields ? 2 * nd_u : 0) + 1) & ~1;
+  z.ew = ((nslot * NQ3 + 1) & ~1) + z.dw;
+  z.table_words = (size_t)sf_table_words(nd_u, nd_m, fields, NQ) + 4 * NT * NC * NQ;
+  z.elem_words = (size_t)z.m_max * z.ew;
+  const size_t s2d = (size_t)z.sx_max * z.sy_max;
+  // even word counts, so that the staged maps after the planes start 16-byte aligned
+  z.plane0_words = (s2d * (2 * pc + 1) + 1) & ~(size_t)1;
+  z.plane1_words = (s2d * (pt == 2 ? pc + 1 : 2 * pc + 1) + 1) & ~(size_t)1;
+  z.sig_bytes = ((s2d * (2 * pc + 1) + 15) & ~(size_t)15);
+  z.bytes = (z.table_words + z.elem_words + z.plane0_words + z.plane1_words) * sizeof(double) +
+            FU_SLOTS * z.sig_bytes + FU_SLOTS * (size_t)z.rows_max * sizeof(int2) + (size_t)(pt * nz_max + 1) * 8;
+  return z;
+}
+
+template <typename CT, int NT, int NC, int QP, int NQ, typename OT, int BS>
+__global__ void __launch_bounds__(BS) k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G,
+                                              FusedSizes Z, const double* __restrict__ t1t,
+                                              const double* __restrict__ t1c, OT* __restrict__ data) {
+  constexpr int pt = NT - 1, pc = NC - 1;
+  constexpr int NQ2 = NQ * NQ, NQ3 = NQ2 * NQ;
+  constexpr int NWAVE = BS / 64;
+  extern __shared__ double lds[];
+  __shared__ int32_t UT[32];
+  __shared__ int32_t TASK[FU_MAXT];
+  __shared__ int32_t B2D[FU_MAXL * FU_MAXL];
+  __shared__ int32_t WXs[FU_MAXL], WYs[FU_MAXL];
+  __shared__ int32_t EFLAG[FU_MAXM];
+  __shared__ int32_t TCOUNT[FU_MAXM + 1];
+  __shared__ uint8_t ROWOF[FU_MAXS2D];  // the row of every stencil place of a unit-height plane
+
+  const int nu = P.nd_u, nm = P.nd_m;
+  const bool fields = SP.need_fields != 0;
+  // the thread's index, made opaque again at the start of every phase: per-thread quantities
+  // derived from it are recomputed there rather than kept live across the contraction, whose
+  // registers they would otherwise spill to scratch memory (a global-latency reload inside
+  // every barrier-bound step)
+  int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  auto rederive = [&]() {
+    int t = (int)threadIdx.x;
+    asm volatile("" : "+v"(t));
+    tid = t;
+    wave = t >> 6;
+    lane = t & 63;
+  };
+  // ------------------------------------------------------------------ the tile
+  const int ntile = G.ntx * G.nty;
+  const int tile = blockIdx.x % ntile, chunk = blockIdx.x / ntile;
+  const int X = tile % G.ntx, Y = tile / G.ntx;
+  const int x0 = X * G.bx, x1 = min(x0 + G.bx, G.nx);
+  const int y0 = Y * G.by, y1 = min(y0 + G.by, G.ny);
+  const int z0 = chunk * G.lz_chunk, z1 = min(z0 + G.lz_chunk, G.nz);
+  const int xs = x0 > 0 ? x0 - 1 : 0, ys = y0 > 0 ? y0 - 1 : 0, zs = z0 > 0 ? z0 - 1 : 0;
+  const int cx = x1 - xs, cy = y1 - ys, M = cx * cy;
+  const int lx_lo = pt * x0, lx_hi = pt * x1 + (x1 == G.nx ? 1 : 0);
+  const int ly_lo = pt * y0, ly_hi = pt * y1 + (y1 == G.ny ? 1 : 0);
+  const int lz_lo = pt * z0, lz_hi = pt * z1 + (z1 == G.nz ? 1 : 0);
+  const int cxl = lx_hi - lx_lo, cyl = ly_hi - ly_lo, nrows = cxl * cyl;
+  const int64_t plane_rows = (int64_t)(pt * G.nx + 1) * (pt * G.ny + 1);
+
+  // ------------------------------------------------------------------ LDS
+  double* TX = lds;
+  double* TM = TX + NQ3 * 24;
+  double* TW = TM + NQ3 * nm;
+  double* T1U = TW + NQ3;
+  double* PT = lds + sf_table_words(nu, nm, fields, NQ);
+  double* EL = lds + Z.table_words;                  // element regions
+  double* BUF0 = EL + Z.elem_words;                  // plane buffers
+  double* BUF1 = BUF0 + Z.plane0_words;
+  uint8_t* SIG = (uint8_t*)(BUF1 + Z.plane1_words);  // staged stencil maps, FU_SLOTS planes
+  int2* HDR = (int2*)(SIG + FU_SLOTS * Z.sig_bytes); // staged row headers, FU_SLOTS planes
+  int64_t* SOFF = (int64_t*)(HDR + FU_SLOTS * Z.rows_max);  // the chunk's planes' map starts
+  const int wpair = (G.nslot * NQ3 + 1) & ~1;
+  auto Wreg = [&](int m) { return EL + (size_t)m * Z.ew; };
+  auto Dreg = [&](int m) { return EL + (size_t)m * Z.ew + wpair; };
+
+  {
+    for (int i = tid; i < 4 * NT * NC * NQ; i += BS) {
+      const int q = i % NQ, b = (i / NQ) % NC, a = (i / (NQ * NC)) % NT, dd = i / (NQ * NC * NT);
+      PT[i] = t1t[((dd >> 1) * NT + a) * NQ + q] * t1c[((dd & 1) * NC + b) * NQ + q];
+    }
+    for (int i = tid; i < NQ3 * 24; i += BS) TX[(i % 24) * NQ3 + i / 24] = P.tab_x[i];
+    for (int i = tid; i < NQ3 * nm; i += BS) TM[(i % nm) * NQ3 + i / nm] = P.tab_m[i];
+    for (int i = tid; i < NQ3; i += BS) TW[i] = P.wq[i];
+    if (fields) {
+      for (int i = tid; i < 2 * P.n1_u * NQ; i += BS) T1U[i] = P.tab1_u[i];
+      for (int i = tid; i < nu; i += BS) UT[P.lex_u[i]] = i;
+    }
+    for (size_t i = tid; i < Z.plane0_words + Z.plane1_words; i += BS) BUF0[i] = 0.0;
+    for (int i = tid; i < lz_hi - lz_lo; i += BS) SOFF[i] = G.sig_off[(int64_t)(lz_lo + i) * ntile + tile];
+    // stencil widths of the tile's lattice points, and every row's place in a plane buffer
+    for (int i = tid; i < cxl; i += BS) WXs[i] = fu_width(lx_lo + i, pt, G.nx, pc);
+    for (int i = tid; i < cyl; i += BS) WYs[i] = fu_width(ly_lo + i, pt, G.ny, pc);
+    // contraction tasks of each element slot: (a1, a2) of its owned rows, every b1
+    for (int m = tid; m < M; m += BS) {
+      const int ex = xs + m % cx, ey = ys + m / cx;
+      const int a1n = min(lx_hi - pt * ex, NT) - max(lx_lo - pt * ex, 0);
+      const int a2n = min(ly_hi - pt * ey, NT) - max(ly_lo - pt * ey, 0);
+      TCOUNT[m + 1] = a1n * a2n * NC;
+    }
+  }
+  const int64_t hdr0 = G.hdr_off[tile];  // the tile's first row in every plane (plane 0's)
+  __syncthreads();
+  if (tid == 0) {
+    int sx = 0, px[FU_MAXL];
+    for (int i = 0; i < cxl; ++i) {
+      px[i] = sx;
+      sx += WXs[i];
+    }
+    int py = 0;
+    for (int iy = 0; iy < cyl; ++iy) {
+      for (int ix = 0; ix < cxl; ++ix) B2D[iy * cxl + ix] = sx * py + WYs[iy] * px[ix];
+      py += WYs[iy];
+    }
+    TCOUNT[0] = 0;
+    for (int m = 0; m < M; ++m) TCOUNT[m + 1] += TCOUNT[m];
+  }
+  __syncthreads();
+  for (int m = tid; m < M; m += BS) {
+    const int ex = xs + m % cx, ey = ys + m / cx;
+    const int a1lo = max(lx_lo - pt * ex, 0), a1hi = min(lx_hi - pt * ex, NT);
+    const int a2lo = max(ly_lo - pt * ey, 0), a2hi = min(ly_hi - pt * ey, NT);
+    int k = TCOUNT[m];
+    for (int a2 = a2lo; a2 < a2hi; ++a2)
+      for (int a1 = a1lo; a1 < a1hi; ++a1)
+        for (int b1 = 0; b1 < NC; ++b1) TASK[k++] = m | (a1 << 8) | (b1 << 12) | (a2 << 16);
+  }
+  for (int rho = tid; rho < nrows; rho += BS) {
+    const int b = B2D[rho], w = WXs[rho % cxl] * WYs[rho / cxl];
+    for (int q = 0; q < w; ++q) ROWOF[b + q] = (uint8_t)rho;
+  }
+  __syncthreads();
+  const int ntask = TCOUNT[M];
+  const int s2d_row = B2D[nrows - 1] + WXs[cxl - 1] * WYs[cyl - 1];  // stencil entries / wz
+
+  // ------------------------------------------------------------------ element data
+  // items of a layer: per element slot 8 vertices, nd_m parameter dofs, and (fields) nd_u
+  // state/adjoint dofs; thread tid takes items tid, tid + BS, ...  A layer's data is loaded
+  // into registers during the layer before (after its contraction, so that the loads are in
+  // flight during its adds and are not queued behind its stores) and put in LDS at the top of
+  // its own layer.
+  const int NI = 8 + nm + (fields ? nu : 0);
+  int32_t nid[FU_ITEMS];
+  double nv[FU_ITEMS][3];
+  auto item_elem = [&](int it, int ez) -> int64_t {
+    const int m = it / NI;
+    const int ex = xs + m % cx, ey = ys + m / cx;
+    return (int64_t)ex + (int64_t)G.nx * ((int64_t)ey + (int64_t)G.ny * ez);
+  };
+  auto load_ids = [&](int ez) {
+#pragma unroll
+    for (int k = 0; k < FU_ITEMS; ++k) {
+      const int it = tid + k * BS;
+      if (it >= M * NI || ez >= z1) continue;
+      const int64_t e = item_elem(it, ez);
+      const int w = it % NI;
+      if (w < 8)
+        nid[k] = P.elem_verts[e * 8 + w];
+      else if (w < 8 + nm)
+        nid[k] = P.edofs_m[e * nm + (w - 8)];
+      else
+        nid[k] = P.edofs_u[e * nu + (w - 8 - nm)];
+    }
+  };
+  auto load_values = [&](int ez) {
+#pragma unroll
+    for (int k = 0; k < FU_ITEMS; ++k) {
+      const int it = tid + k * BS;
+      if (it >= M * NI || ez >= z1) continue;
+      const int w = it % NI;
+      const int64_t d = nid[k];
+      if (w < 8) {
+        nv[k][0] = P.verts[d * 3];
+        nv[k][1] = P.verts[d * 3 + 1];
+        nv[k][2] = P.verts[d * 3 + 2];
+      } else if (w < 8 + nm) {
+        nv[k][0] = P.m[d];
+      } else {
+        nv[k][0] = P.u[d];
+        nv[k][1] = P.p[d];
+      }
+    }
+  };
+  auto store_values = [&](int ez) {
+#pragma unroll
+    for (int k = 0; k < FU_ITEMS; ++k) {
+      const int it = tid + k * BS;
+      if (it >= M * NI || ez >= z1) continue;
+      const int m = it / NI, w = it % NI;
+      double* D = Dreg(m);
+      if (w < 8) {
+        D[w * 3] = nv[k][0];
+        D[w * 3 + 1] = nv[k][1];
+        D[w * 3 + 2] = nv[k][2];
+      } else if (w < 8 + nm) {
+        D[24 + (w - 8)] = nv[k][0];
+      } else {
+        const int a = UT[w - 8 - nm];
+        D[24 + nm + a] = nv[k][0];
+        D[24 + nm + nu + a] = nv[k][1];
+      }
+    }
+  };
+
+  // ------------------------------------------------------------------ planes written out
+  // the planes a layer completes: z = p_t ez + a3 for a3 < p_t that the chunk owns (staged
+  // slots 0, 1), and at the top of the box the plane z = p_t n_z after the last layer (staged
+  // in slot 0 once the first plane of that layer is out)
+  auto retire_plane = [&](int ez, int k) -> int {
+    int n = 0;
+    for (int a3 = 0; a3 < pt; ++a3) {
+      const int lz = pt * ez + a3;
+      if (lz >= lz_lo && lz < lz_hi) {
+        if (n == k) return lz;
+        ++n;
+      }
+    }
+    return -1;
+  };
+  auto wz_of = [&](int lz) { return fu_width(lz, pt, G.nz, pc); };
+  // row headers and stencil maps of a layer's planes, in registers between loads and LDS stores
+  int2 hv[FU_SLOTS];
+  uint4 sv[FU_SIGV];
+  auto load_planes = [&](int ez) {
+    int lzs[FU_SLOTS], chunks[FU_SLOTS];
+#pragma unroll
+    for (int k = 0; k < FU_SLOTS; ++k) {
+      lzs[k] = ez < z1 ? retire_plane(ez, k) : -1;
+      chunks[k] = lzs[k] >= 0 ? (s2d_row * wz_of(lzs[k]) + 15) >> 4 : 0;
+      if (lzs[k] >= 0 && tid < nrows) hv[k] = G.hdr[hdr0 + (int64_t)lzs[k] * plane_rows + tid];
+    }
+#pragma unroll
+    for (int v = 0; v < FU_SIGV; ++v) {
+      const int c = tid + v * BS;
+      sv[v] = make_uint4(0, 0, 0, 0);
+      if (c < chunks[0])
+        sv[v] = ((const uint4*)(G.sigma + SOFF[lzs[0] - lz_lo]))[c];
+      else if (c - chunks[0] < chunks[1])
+        sv[v] = ((const uint4*)(G.sigma + SOFF[lzs[1] - lz_lo]))[c - chunks[0]];
+    }
+  };
+  auto store_planes = [&](int ez) {
+    int lzs[FU_SLOTS], chunks[FU_SLOTS];
+#pragma unroll
+    for (int k = 0; k < FU_SLOTS; ++k) {
+      lzs[k] = ez < z1 ? retire_plane(ez, k) : -1;
+      chunks[k] = lzs[k] >= 0 ? (s2d_row * wz_of(lzs[k]) + 15) >> 4 : 0;
+      if (lzs[k] >= 0 && tid < nrows) HDR[k * Z.rows_max + tid] = hv[k];
+    }
+#pragma unroll
+    for (int v = 0; v < FU_SIGV; ++v) {
+      const int c = tid + v * BS;
+      if (c < chunks[0])
+        ((uint4*)SIG)[c] = sv[v];
+      else if (c - chunks[0] < chunks[1])
+        ((uint4*)(SIG + Z.sig_bytes))[c - chunks[0]] = sv[v];
+    }
+  };
+  // the top plane's tables into slot 0, directly (once a chunk, at the top of the box)
+  auto stage_top = [&](int lz) {
+    if (tid < nrows) HDR[tid] = G.hdr[hdr0 + (int64_t)lz * plane_rows + tid];
+    const int chunks = (s2d_row * wz_of(lz) + 15) >> 4;
+    const uint4* src = (const uint4*)(G.sigma + SOFF[lz - lz_lo]);
+    for (int c = tid; c < chunks; c += BS) ((uint4*)SIG)[c] = src[c];
+  };
+  // every row of a staged plane as whole CSR rows, its buffer cleared: thread tid takes the
+  // plane's places tid, tid + BS, ... (in row order, so the stores of a row are contiguous),
+  // every read of the batch issued before its first use.  A place's row: ROWOF[k / wz] (a row
+  // of the plane holds w_x w_y wz places from B2D wz on); k / wz by a multiply and a shift
+  // (exact for k < 2^16 / wz).
+  auto write_plane = [&](int lz, int k) {
+    rederive();
+    const int wz = wz_of(lz);
+    const int S = s2d_row * wz;
+    const uint32_t mul = (65536u + (uint32_t)wz - 1u) / (uint32_t)wz;
+    double* buf = (lz & 1) ? BUF1 : BUF0;
+    const uint8_t* sg = SIG + k * Z.sig_bytes;
+    const int2* hdr = HDR + k * Z.rows_max;
+    for (int base = 0; base < S; base += FU_EB * BS) {
+      int kk[FU_EB], rb[FU_EB];
+      unsigned c[FU_EB];
+      int2 hd[FU_EB];
+      double v[FU_EB];
+#pragma unroll
+      for (int e = 0; e < FU_EB; ++e) {
+        kk[e] = base + tid + e * BS;
+        const bool ok = kk[e] < S;
+        const int rho = ok ? ROWOF[((uint32_t)kk[e] * mul) >> 16] : 0;
+        c[e] = ok ? sg[kk[e]] : 0u;
+        hd[e] = hdr[rho];
+        rb[e] = B2D[rho] * wz;
+      }
+#pragma unroll
+      for (int e = 0; e < FU_EB; ++e) v[e] = kk[e] < S ? buf[rb[e] + (c[e] & 0x7F)] : 0.0;
+#pragma unroll
+      for (int e = 0; e < FU_EB; ++e)
+        if (kk[e] < S) {
+          buf[rb[e] + (c[e] & 0x7F)] = 0.0;
+          const int t = kk[e] - rb[e];
+          double o = (c[e] & 0x80) ? 0.0 : v[e];
+          if (hd[e].y != 0) o = (hd[e].y == 2 && t == 0) ? G.diag : 0.0;
+          data[(int64_t)hd[e].x + t] = (OT)o;
+        }
+    }
+  };
+
+  // ------------------------------------------------------------------ the sweep
+  const uint32_t every = SP.npair >= 32 ? ~0u : (1u << SP.npair) - 1u;
+  bool bad = false;
+  CT acc[NC][NT][NC];
+  load_ids(zs);
+  load_values(zs);
+  load_ids(zs + 1);
+  for (int ez = zs; ez < z1; ++ez) {
+    rederive();
+    // 1. this layer's element data, loaded during the layer before; the tables of the planes
+    // it completes in flight during the point phase
+    store_values(ez);
+    __syncthreads();
+    load_planes(ez);
+    // each element's alignment (every vertex coordinate that of the vertex with the same index
+    // along its axis: J diagonal, the pairs of two different derivatives left out of its
+    // contraction, as the element kernel does)
+    for (int m = tid; m < M; m += BS) {
+      const double* Xe = Dreg(m);
+      bool al = true;
+      for (int v = 0; v < 8; ++v)
+        al = al && Xe[v * 3] == Xe[(v & 1) * 3] && Xe[v * 3 + 1] == Xe[(v & 2) * 3 + 1] &&
+             Xe[v * 3 + 2] == Xe[(v & 4) * 3 + 2];
+      EFLAG[m] = al ? 1 : 0;
+    }
+    __syncthreads();
+    // 2. (element, point) pairs: geometry, fields, pointwise pairs (one LDS slot for the pairs
+    // of equal values)
+    for (int t = tid; t < M * NQ3; t += BS) {
+      const int m = t / NQ3, q = t - m * NQ3;
+      const double* D = Dreg(m);
+      double* We = Wreg(m);
+      PointState S;
+      sf_point_state<NQ>(P, q, D, D + 24 + nm, D + 24, D + 24 + nm + nu, fields, TX, TM, TW, T1U, S,
+                         EFLAG[m] != 0);
+      double Db[4][4];
+      point_block(s, j, S, Db);
+      int p = 0;
+#pragma unroll
+      for (int k = 0; k < 16; ++k)
+        if ((SP.kmask >> k) & 1) {
+          if (G.wfirst & (1u << p)) We[G.ws[p] * NQ3 + q] = Db[k >> 2][k & 3];
+          ++p;
+        }
+    }
+    store_planes(ez);
+    __syncthreads();
+    // 3. the contraction of this thread's task
+    rederive();
+    bool has = tid < ntask;
+    int m = 0, a1 = 0, b1 = 0, a2 = 0;
+    if (has) {
+      const int tk = TASK[tid];
+      m = tk & 0xFF;
+      a1 = (tk >> 8) & 0xF;
+      b1 = (tk >> 12) & 0xF;
+      a2 = (tk >> 16) & 0xF;
+    }
+#pragma unroll
+    for (int i = 0; i < NC; ++i)
+#pragma unroll
+      for (int a3 = 0; a3 < NT; ++a3)
+#pragma unroll
+        for (int b3 = 0; b3 < NC; ++b3) acc[i][a3][b3] = 0;
+    if (has) {
+      const uint32_t live = every & (EFLAG[m] ? ~(uint32_t)SP.offdiag : ~0u);
+      const double* Wl = Wreg(m);
+#pragma unroll 1
+      for (int t = 0; t < 4; ++t) {
+        const uint32_t tm = live & (uint32_t)SP.tmask[t];
+        if (tm == 0) continue;
+        CT s2[NC][NQ];
+#pragma unroll
+        for (int i = 0; i < NC; ++i)
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) s2[i][q] = 0;
+        for (int p = 0; p < SP.npair; ++p) {
+          if (!((tm >> p) & 1)) continue;
+          const int d1 = ((SP.pr[p] == 1) << 1) | (SP.pc[p] == 1);
+          const int d2 = ((SP.pr[p] == 2) << 1) | (SP.pc[p] == 2);
+          const double* wv = Wl + G.ws[p] * NQ3;
+          const double* f1 = PT + ((d1 * NT + a1) * NC + b1) * NQ;
+          double gd[NQ];
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) gd[q] = f1[q];
+          double hd[NC][NQ];
+#pragma unroll
+          for (int i = 0; i < NC; ++i) {
+            const double* f2 = PT + ((d2 * NT + a2) * NC + i) * NQ;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) hd[i][q] = f2[q];
+          }
+          __builtin_amdgcn_sched_barrier(0);
+          CT g[NQ];
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) g[q] = (CT)gd[q];
+#pragma unroll
+          for (int q3 = 0; q3 < NQ; ++q3) {
+            CT s1[NQ];
+#pragma unroll
+            for (int q2 = 0; q2 < NQ; ++q2) {
+              const double* wr = wv + NQ * q2 + NQ2 * q3;
+              if constexpr (NQ == 3) {
+                s1[q2] = fma(g[2], (CT)wr[2], fma(g[1], (CT)wr[1], g[0] * (CT)wr[0]));
+              } else {
+                CT a = g[0] * (CT)wr[0];
+#pragma unroll
+                for (int q1 = 1; q1 < NQ; ++q1) a = fma(g[q1], (CT)wr[q1], a);
+                s1[q2] = a;
+              }
+            }
+            if constexpr (NQ == 3) {
+#pragma unroll
+              for (int i = 0; i < NC; ++i)
+                s2[i][q3] = fma((CT)hd[i][2], s1[2], fma((CT)hd[i][1], s1[1], fma((CT)hd[i][0], s1[0], s2[i][q3])));
+            } else {
+#pragma unroll
+              for (int i = 0; i < NC; ++i) {
+                CT a = s2[i][q3];
+#pragma unroll
+                for (int q2 = 0; q2 < NQ; ++q2) a = fma((CT)hd[i][q2], s1[q2], a);
+                s2[i][q3] = a;
+              }
+            }
+          }
+        }
+        const double* bt = t1t + (t >> 1) * NT * NQ;
+        const double* bc = t1c + (t & 1) * NC * NQ;
+#pragma unroll
+        for (int b3 = 0; b3 < NC; ++b3) {
+          CT u[NC][NQ];
+#pragma unroll
+          for (int i = 0; i < NC; ++i)
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) u[i][q] = (CT)bc[b3 * NQ + q] * s2[i][q];
+#pragma unroll
+          for (int a3 = 0; a3 < NT; ++a3) {
+            if constexpr (NQ == 3) {
+              const CT k0 = (CT)bt[a3 * 3], k1 = (CT)bt[a3 * 3 + 1], k2 = (CT)bt[a3 * 3 + 2];
+#pragma unroll
+              for (int i = 0; i < NC; ++i)
+                acc[i][a3][b3] = fma(k2, u[i][2], fma(k1, u[i][1], fma(k0, u[i][0], acc[i][a3][b3])));
+            } else {
+              CT k[NQ];
+#pragma unroll
+              for (int q = 0; q < NQ; ++q) k[q] = (CT)bt[a3 * NQ + q];
+#pragma unroll
+              for (int i = 0; i < NC; ++i) {
+                CT a = acc[i][a3][b3];
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) a = fma(k[q], u[i][q], a);
+                acc[i][a3][b3] = a;
+              }
+            }
+          }
+        }
+      }
+    }
+    // the next layer's element data and planes' tables in flight during the adds
+    rederive();
+    load_values(ez + 1);
+    load_ids(ez + 2);
+    // the task again, from LDS (nothing of it kept live through the contraction)
+    has = tid < ntask;
+    {
+      const int tk = has ? TASK[tid] : 0;
+      m = tk & 0xFF;
+      a1 = (tk >> 8) & 0xF;
+      b1 = (tk >> 12) & 0xF;
+      a2 = (tk >> 16) & 0xF;
+    }
+    // 4./5. the adds in ascending element order, and the planes written out.  Phase A takes the
+    // rows of a3 = 0 and 1 (two planes in two buffers), phase B those of a3 = 2 (p_t = 2; into
+    // the buffer of a3 = 0, cleared when that plane was written out).  Within a phase the four
+    // sub-steps of local (y, x) order the adds of every place by element; the places of one
+    // lane are distinct, so a lane reads all of them before it writes any.
+    const int ex = xs + m % cx, ey = ys + m / cx;
+    const int ix = pt * ex + a1 - lx_lo, iy = pt * ey + a2 - ly_lo;
+    const int grp = (a2 >= 1 ? 0 : 2) + (a1 >= 1 ? 0 : 1);
+    const int ox = pc * ((a1 == 0 && ex > 0) ? 1 : 0) + b1;
+    const int oy0 = pc * ((a2 == 0 && ey > 0) ? 1 : 0);
+    const int wx = has ? WXs[ix] : 0, wy = has ? WYs[iy] : 0;
+    const int b2d = has ? B2D[iy * cxl + ix] : 0;
+    int rk = 0;  // staged planes written so far
+    constexpr int NA = NT < 2 ? NT : 2;  // rows of phase A: a3 < NA
+#pragma unroll
+    for (int ph = 0; ph < 2; ++ph) {
+      const int a3lo = ph == 0 ? 0 : NA, a3hi = ph == 0 ? NA : NT;
+      if (a3lo >= a3hi) continue;
+      bool any = false;  // the phase holds a plane the chunk owns (uniform over the block)
+      for (int a3 = a3lo; a3 < a3hi; ++a3) {
+        const int lz = pt * ez + a3;
+        any = any || (lz >= lz_lo && lz < lz_hi);
+      }
+      if (!any) continue;
+#pragma unroll 1
+      for (int g = 0; g < 4; ++g) {
+        if (has && grp == g) {
+#pragma unroll
+          for (int a3 = a3lo; a3 < a3hi; ++a3) {
+            const int lz = pt * ez + a3;
+            if (lz < lz_lo || lz >= lz_hi) continue;  // a row of the chunk below (halo)
+            const int wz = wz_of(lz);
+            double* row = ((lz & 1) ? BUF1 : BUF0) + (size_t)b2d * wz;
+            const int oz0 = pc * ((a3 == 0 && ez > 0) ? 1 : 0);
+            double old[NC][NC];
+#pragma unroll
+            for (int i = 0; i < NC; ++i)
+#pragma unroll
+              for (int b3 = 0; b3 < NC; ++b3) old[i][b3] = row[ox + wx * ((oy0 + i) + wy * (oz0 + b3))];
+#pragma unroll
+            for (int i = 0; i < NC; ++i)
+#pragma unroll
+              for (int b3 = 0; b3 < NC; ++b3) {
+                const CT v = acc[i][a3][b3];
+                bad |= !__builtin_isfinite(v);
+                row[ox + wx * ((oy0 + i) + wy * (oz0 + b3))] = old[i][b3] + (double)v;
+              }
+          }
+        }
+        __syncthreads();
+      }
+      // the planes of this phase no later layer adds to
+      for (int a3 = a3lo; a3 < a3hi; ++a3) {
+        const int lz = pt * ez + a3;
+        if (lz < lz_lo || lz >= lz_hi) continue;
+        if (a3 < pt) {
+          write_plane(lz, rk++);
+          __syncthreads();
+        } else if (ez == G.nz - 1 && z1 == G.nz) {
+          // the top of the box: its tables into slot 0, then written out
+          rederive();
+          stage_top(lz);
+          __syncthreads();
+          write_plane(lz, 0);
+          __syncthreads();
+        }
+      }
+    }
+  }
+  if (bad && P.nonfinite) atomicOr(P.nonfinite, 1);
+}
+
+template <int NT, int NC, int NQ>
+static FusedSizes fu_pick(const hm_problem& P, const SumfactPlan& SP, int nslot, int bs, size_t limit, int* bx,
+                          int* by) {
+  static const int cand[][2] = {{4, 4}, {4, 3}, {3, 3}, {3, 2}, {2, 2}, {2, 1}, {1, 1}};
+  FusedSizes z{};
+  for (const auto& c : cand) {
+    z = fu_sizes<NT, NC, NQ>(c[0], c[1], nslot, P.nd_u, P.nd_m, SP.need_fields != 0, FU_MAXZ);
+    const int items = (z.m_max * (8 + P.nd_m + (SP.need_fields ? P.nd_u : 0)) + bs - 1) / bs;
+    const int sigv = (int)((FU_SLOTS * z.sig_bytes / 16 + bs - 1) / bs);
+    if (z.bytes <= limit && z.t_max <= bs && z.t_max <= FU_MAXT && z.m_max <= FU_MAXM && items <= FU_ITEMS &&
+        sigv <= FU_SIGV && z.rows_max <= bs && z.rows_max <= 256 && (size_t)z.sx_max * z.sy_max <= FU_MAXS2D &&
+        z.sig_bytes <= 8192 && (NT - 1) * c[0] + 1 <= FU_MAXL && (NT - 1) * c[1] + 1 <= FU_MAXL) {
+      *bx = c[0];
+      *by = c[1];
+      return z;
+    }
+  }
+  *bx = *by = 0;
+  return z;
+}
+
+// the static LDS of k_fused (task table, row places, widths, flags), left out of its dynamic LDS
+constexpr size_t FU_STATIC_LDS =
+    sizeof(int32_t) * (32 + FU_MAXT + FU_MAXL * FU_MAXL + 2 * FU_MAXL + 2 * FU_MAXM + 1) + FU_MAXS2D;
+
+static size_t lds_limit() {
+  int dev = 0, v = 0;
+  size_t lim = 64 * 1024;
+  if (hipGetDevice(&dev) == hipSuccess &&
+      hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+    lim = (size_t)v;
+  return lim > FU_STATIC_LDS ? lim - FU_STATIC_LDS : 0;
+}
+
+// The LDS slots of the pointwise pairs: a pair whose values are those of an earlier one takes
+// its slot (the reference-gradient pairs (k, l) and (l, k), whose weights w g J^-1 J^-T are
+// symmetric to the bit, and every (r, c), (c, r) of a block of a slot with itself).
+struct FuSlots {
+  int ws[SF_MAXPAIR];
+  int nslot;
+  uint32_t wfirst;
+};
+
+static FuSlots fu_slots(const SumfactPlan& SP, int i, int j) {
+  FuSlots W{};
+  for (int p = 0; p < SP.npair; ++p) {
+    const int r = SP.pr[p], c = SP.pc[p];
+    int q = -1;
+    if (r > c && ((r >= 1 && c >= 1) || i == j))
+      for (int t = 0; t < p; ++t)
+        if (SP.pr[t] == c && SP.pc[t] == r) q = t;
+    if (q >= 0) {
+      W.ws[p] = W.ws[q];
+    } else {
+      W.ws[p] = W.nslot++;
+      W.wfirst |= 1u << p;
+    }
+  }
+  return W;
+}
+
+// The pairs and fields of block (i, j), as the element-matrix kernel plans them.
+static int fu_plan(const hm_problem* P, int i, int j, SumfactPlan& SP, int* nt, int* nc) {
+  if (int rc = check_tensor(P)) return rc;
+  *nt = i == HM_PARAMETER ? P->n1_m : P->n1_u;
+  *nc = j == HM_PARAMETER ? P->n1_m : P->n1_u;
+  if (*nt < 2 || *nt > 3 || *nc < 2 || *nc > 3) HM_FAIL(HM_ERR_UNSUPPORTED, "1-D orders outside 1..2");
+  if (*nt < *nc) HM_FAIL(HM_ERR_UNSUPPORTED, "fused assembly takes test orders of at least the trial order");
+  SP = SumfactPlan{};
+  const bool nl = P->density == HM_DENSITY_NONLINEAR;
+  SP.npair = block_pairs(i, j, nl, SP.pr, SP.pc, 0, &SP.kmask);
+  if (SP.npair == 0) HM_FAIL(HM_ERR_UNSUPPORTED, "block (%d, %d) is zero for this density", i, j);
+  plan_masks(SP);
+  SP.need_fields = !(i * 3 + j == 6 && !nl);
+  if (SP.need_fields && (P->n1_u < 2 || P->n1_u > 3))
+    HM_FAIL(HM_ERR_UNSUPPORTED, "state fields of 1-D order outside 1..2");
+  return HM_OK;
+}
+
+template <int NT, int NC, int NQ>
+static int fu_tile(const hm_problem* P, const SumfactPlan& SP, int nslot, int* bx, int* by) {
+  constexpr int BS = NQ == 3 ? 768 : 512;
+  fu_pick<NT, NC, NQ>(*P, SP, nslot, BS, lds_limit(), bx, by);
+  if (*bx == 0) HM_FAIL(HM_ERR_UNSUPPORTED, "no tile of the fused assembly fits the LDS");
+  return HM_OK;
+}
+
+template <typename CT, int NT, int NC, int QP, int NQ, typename OT>
+static int fu_launch(const hm_problem& P, int i, int j, const SumfactPlan& SP, const hm_fused_plan& F, void* data,
+                     hipStream_t st) {
+  constexpr int BS = NQ == 3 ? 768 : 512;
+  const FuSlots W = fu_slots(SP, i, j);
+  int bx = 0, by = 0;
+  const FusedSizes Z = fu_pick<NT, NC, NQ>(P, SP, W.nslot, BS, lds_limit(), &bx, &by);
+  if (bx != F.bx || by != F.by)
+    HM_FAIL(HM_ERR_ARG, "the plan is laid out for tiles of %d x %d, this block takes %d x %d", F.bx, F.by, bx, by);
+  auto kern = k_fused<CT, NT, NC, QP, NQ, OT, BS>;
+  if (Z.bytes > 48 * 1024) {
+    const hipError_t err = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               (int)Z.bytes);
+    if (err != hipSuccess) HM_FAIL(HM_ERR_HIP, "hipFuncSetAttribute: %s", hipGetErrorString(err));
+  }
+  FusedGeom G{};
+  G.nx = F.nx;
+  G.ny = F.ny;
+  G.nz = F.nz;
+  G.bx = bx;
+  G.by = by;
+  G.ntx = (F.nx + bx - 1) / bx;
+  G.nty = (F.ny + by - 1) / by;
+  G.hdr = (const int2*)F.hdr;
+  G.hdr_off = F.hdr_off;
+  G.sigma = F.sigma;
+  G.sig_off = F.sig_off;
+  G.diag = F.diag;
+  for (int p = 0; p < SF_MAXPAIR; ++p) G.ws[p] = W.ws[p];
+  G.nslot = W.nslot;
+  G.wfirst = W.wfirst;
+  // layers a chunk: the fewest rounds of blocks over the compute units times the layers a
+  // block sweeps (one more for the halo layer of a cut column)
+  int cus = 256, dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    cus = 256;
+  if (cus < 1) cus = 256;
+  const int64_t ntile = (int64_t)G.ntx * G.nty;
+  int best = 1;
+  int64_t best_cost = INT64_MAX;
+  for (int c = 1; c <= F.nz; ++c) {
+    const int len = (F.nz + c - 1) / c;
+    if (c > 1 && len < 4) break;
+    if (len > FU_MAXZ) continue;
+    const int nch = (F.nz + len - 1) / len;
+    const int64_t rounds = (ntile * nch + cus - 1) / cus;
+    const int64_t cost = rounds * (len + (nch > 1 ? 1 : 0));
+    if (cost < best_cost) {
+      best_cost = cost;
+      best = len;
+    }
+  }
+  G.lz_chunk = best;
+  G.nchunk = (F.nz + best - 1) / best;
+  const int64_t grid = ntile * G.nchunk;
+  hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(BS), Z.bytes, st, P, i, j, SP, G, Z,
+                     (const double*)(i == HM_PARAMETER ? P.tab1_m : P.tab1_u),
+                     (const double*)(j == HM_PARAMETER ? P.tab1_m : P.tab1_u), (OT*)data);
+  return hip_check("k_fused");
+}
+
+template <typename CT, int QP, typename OT>
+static int fu_orders(const hm_problem& P, int i, int j, int nt, int nc, const SumfactPlan& SP,
+                     const hm_fused_plan& F, void* data, hipStream_t st) {
+  if (P.nq1 == 4) {
+    if (nt == 3 && nc == 3) return fu_launch<CT, 3, 3, QP, 4, OT>(P, i, j, SP, F, data, st);
+    if (nt == 3 && nc == 2) return fu_launch<CT, 3, 2, QP, 4, OT>(P, i, j, SP, F, data, st);
+    return fu_launch<CT, 2, 2, QP, 4, OT>(P, i, j, SP, F, data, st);
+  }
+  if (nt == 3 && nc == 3) return fu_launch<CT, 3, 3, QP, 3, OT>(P, i, j, SP, F, data, st);
+  if (nt == 3 && nc == 2) return fu_launch<CT, 3, 2, QP, 3, OT>(P, i, j, SP, F, data, st);
+  return fu_launch<CT, 2, 2, QP, 3, OT>(P, i, j, SP, F, data, st);
+}
+
+}  // namespace hm
+
+using namespace hm;
+
+extern "C" int hm_fused_tile(const hm_problem* P, int i, int j, int precision, int32_t* bx, int32_t* by) {
+  HM_REQUIRE(P && bx && by, "bad fused tile arguments");
+  if (precision != HM_FP64 && precision != HM_FP32)
+    HM_FAIL(HM_ERR_UNSUPPORTED, "fused assembly in FP64 or FP32 only");
+  SumfactPlan SP;
+  int nt = 0, nc = 0;
+  if (int rc = fu_plan(P, i, j, SP, &nt, &nc)) return rc;
+  int x = 0, y = 0, rc = HM_OK;
+  const int ns = fu_slots(SP, i, j).nslot;
+  if (P->nq1 == 4) {
+    rc = nt == 3 && nc == 3 ? fu_tile<3, 3, 4>(P, SP, ns, &x, &y)
+         : nt == 3         ? fu_tile<3, 2, 4>(P, SP, ns, &x, &y)
+                           : fu_tile<2, 2, 4>(P, SP, ns, &x, &y);
+  } else {
+    rc = nt == 3 && nc == 3 ? fu_tile<3, 3, 3>(P, SP, ns, &x, &y)
+         : nt == 3         ? fu_tile<3, 2, 3>(P, SP, ns, &x, &y)
+                           : fu_tile<2, 2, 3>(P, SP, ns, &x, &y);
+  }
+  *bx = x;
+  *by = y;
+  return rc;
+}
+
+extern "C" int hm_assemble_fused(const hm_problem* P, int i, int j, const hm_fused_plan* F, int precision,
+                                 void* data, int out_dtype, void* stream) {
+  HM_REQUIRE(P && F && data, "bad fused assembly arguments");
+  HM_REQUIRE(F->nx >= 1 && F->ny >= 1 && F->nz >= 1 && (int64_t)F->nx * F->ny * F->nz == P->ne,
+             "a box of %d x %d x %d elements against %lld", F->nx, F->ny, F->nz, (long long)P->ne);
+  HM_REQUIRE(F->hdr && F->hdr_off && F->sigma && F->sig_off, "the fused plan is incomplete");
+  SumfactPlan SP;
+  int nt = 0, nc = 0;
+  if (int rc = fu_plan(P, i, j, SP, &nt, &nc)) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  if (precision == HM_FP64 && out_dtype == HM_FP64)
+    return fu_orders<double, HM_FP64, double>(*P, i, j, nt, nc, SP, *F, data, st);
+  if (precision == HM_FP32 && out_dtype == HM_FP32)
+    return fu_orders<float, HM_FP32, float>(*P, i, j, nt, nc, SP, *F, data, st);
+  HM_FAIL(HM_ERR_UNSUPPORTED, "fused assembly of precision %d into values of type %d", precision, out_dtype);
+}

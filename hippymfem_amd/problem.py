@@ -13,6 +13,7 @@ import torch
 
 from . import _lib
 from ._lib import ADJOINT, PARAMETER, STATE
+from . import fused
 from .csrassemble import _device_plan, assemble_vector_csr, finish_block, plan_block
 from .kernel import QuadratureKernel
 
@@ -67,8 +68,11 @@ def _symmetric(i, j):
 
 
 class BlockAssembly:
-    """Resident buffers and plan of one block: ``assemble`` is one element-kernel launch
-    into ``E`` and one scatter launch into ``data``, both on one stream.
+    """Resident buffers and plan of one block.  ``assemble`` is one launch of the fused
+    kernel (element contraction and scatter in one pass, no element matrix in memory) where
+    the fused route applies (``route="auto"``, the default, or ``"fused"``: a box mesh,
+    tensor elements, FP64 or FP32), else one element-kernel launch into ``E`` and one scatter
+    launch into ``data`` (``route="two-pass"``); both on one stream, with the same bits.
 
     With ``row_sums``, a square block with single-precision values keeps the defect of its
     row sums beside them (``row_defect``, fp64: what the rounding of each row's FP64 sums
@@ -78,8 +82,12 @@ class BlockAssembly:
     rather than made by default here, where no solver reads it."""
 
     def __init__(self, kernel, i, j, test_ess=None, diag_policy="one", precision="fp64",
-                 value_dtype=None, row_sums=False):
+                 value_dtype=None, row_sums=False, route="auto"):
+        if route not in ("auto", "fused", "two-pass"):
+            raise ValueError("route is 'auto', 'fused' or 'two-pass', not %r" % (route,))
         self.kernel, self.i, self.j = kernel, i, j
+        self.route = route
+        self._fused = None
         Vi, Vj = kernel.spaces[i], kernel.spaces[j]
         self.plan = plan_block(Vi, Vj, kernel.batches.groups, test_ess=test_ess,
                                diag_policy=diag_policy, symmetric=_symmetric(i, j))
@@ -107,6 +115,42 @@ class BlockAssembly:
             self._E = torch.empty((k.group.ne, k.nd[self.i], k.nd[self.j]), dtype=self.e_dtype,
                                   device=self.data.device)
         return self._E
+
+    def fused_plan(self):
+        """The plan of the fused route (``fused.FusedPlan``), made at the first use, or None
+        where the block takes the two-pass route."""
+        if self._fused is None:
+            self._fused = False
+            why = None
+            if self.route == "two-pass":
+                why = "two-pass asked for"
+            elif self.row_defect is not None:
+                why = "row sums asked for"
+            elif self.data.dtype != self.e_dtype:
+                why = "values of another precision than the element matrices"
+            elif self.plan.diag_policy == "keep" and self.plan.diag is not None:
+                why = "diag_policy 'keep'"
+            else:
+                why = fused.supported(self.kernel, self.i, self.j, self.precision)
+            tile = None
+            if why is None:
+                tile = fused.tile_of(self.kernel, self.i, self.j, self.precision)
+                if tile is None:
+                    why = "no tile of the fused kernel for this block"
+            if why is None:
+                p = self.plan
+                try:
+                    self._fused = fused.FusedPlan(p.pattern, p.test_space, p.trial_space,
+                                                  self.kernel.batches.mesh, tile,
+                                                  row_mask=p.row_mask, col_mask=p.col_mask,
+                                                  diag=p.diag)
+                except NotImplementedError as exc:
+                    why = str(exc)
+            self.why_two_pass = why
+            if why is not None and self.route == "fused":
+                raise NotImplementedError("block (%d, %d) cannot take the fused route: %s"
+                                          % (self.i, self.j, why))
+        return self._fused or None
 
     def fresh_values(self):
         """An uninitialized values array (and row defect) of this block's shape, for a result
@@ -144,13 +188,35 @@ class BlockAssembly:
         A.row_defect = row_defect
         return A
 
-    def assemble(self, local_arrays, stream=None, out=None, row_defect=None):
-        """Element kernel, then scatter.  The matrix aliases the resident values unless
-        ``out`` (and ``row_defect``) is given: a repeated assembly into resident memory
-        allocates nothing, and the next assembly of this block overwrites it."""
+    def assemble_values(self, local_arrays, stream=None, out=None, row_defect=None):
+        """The block's CSR values at the fields ``local_arrays``, into the resident ``data``
+        or into ``out``: the fused kernel where it applies, else the element kernel then the
+        scatter.  The non-finite flag is set when an element-matrix entry is not finite."""
+        F = self.fused_plan()
+        if F is not None:
+            self.clear_flag(stream)
+            return F.assemble(self.kernel, self.i, self.j, local_arrays, self.precision,
+                              self.data if out is None else out, stream=stream,
+                              nonfinite=self.nonfinite)
         self.element_matrices(local_arrays, stream)
-        self.scatter(stream, out=out, row_defect=row_defect)
+        return self.scatter(stream, out=out, row_defect=row_defect)
+
+    def assemble(self, local_arrays, stream=None, out=None, row_defect=None):
+        """The block at the fields ``local_arrays``.  The matrix aliases the resident values
+        unless ``out`` (and ``row_defect``) is given: a repeated assembly into resident
+        memory allocates nothing, and the next assembly of this block overwrites it."""
+        self.assemble_values(local_arrays, stream, out=out, row_defect=row_defect)
         return self.finish(out, row_defect) if out is not None else self.finish()
+
+    def check_finite(self, local_arrays, what):
+        """``RuntimeError`` where the last pass wrote a non-finite element-matrix entry
+        (``require_finite_arrays``); the element matrices are made for the check when the
+        fused route left none."""
+        if self.finite():
+            return
+        if self.fused_plan() is not None:
+            self.element_matrices(local_arrays)
+        require_finite_arrays([self.E], what)
 
 
 class PDEVariationalProblem:
@@ -158,7 +224,10 @@ class PDEVariationalProblem:
     [Vu, Vm, Vu]`` with essential conditions ``bc0`` (``PDEVariationalProblem``,
     ``PDEVariationalProblem.py:127-197``)."""
 
-    def __init__(self, Vh, density, batches, bc0, is_fwd_linear=None):
+    def __init__(self, Vh, density, batches, bc0, is_fwd_linear=None, route="auto"):
+        """``route``: how the blocks are assembled, ``"auto"`` (the fused kernel where it
+        applies), ``"fused"`` or ``"two-pass"`` (``BlockAssembly``)."""
+        self.route = route
         self.Vh = list(Vh)
         self.density = density
         self.batches = batches
@@ -172,15 +241,17 @@ class PDEVariationalProblem:
         return [x[STATE], x[PARAMETER], x[ADJOINT] if len(x) > ADJOINT and x[ADJOINT] is not None
                 else np.zeros(self.Vh[ADJOINT].ndof)]
 
-    def block_assembly(self, i, j, precision="fp64", value_dtype=None, row_sums=False):
-        key = (i, j, precision, value_dtype) + (("row_sums",) if row_sums else ())
+    def block_assembly(self, i, j, precision="fp64", value_dtype=None, row_sums=False,
+                       route=None):
+        route = self.route if route is None else route
+        key = (i, j, precision, value_dtype, route) + (("row_sums",) if row_sums else ())
         got = self._blocks.get(key)
         if got is None:
             rule = ELIMINATION[(i, j)]
             got = self._blocks[key] = BlockAssembly(
                 self.kernel, i, j, test_ess=self.bc0.ess_tdof if rule["test"] else None,
                 diag_policy=rule["diag_policy"], precision=precision, value_dtype=value_dtype,
-                row_sums=row_sums)
+                row_sums=row_sums, route=route)
         return got
 
     # ------------------------------------------------------------------ assembly
@@ -199,9 +270,9 @@ class PDEVariationalProblem:
         point's blocks, ``PDEVariationalProblem.py:562-568``)."""
         blk = self.block_assembly(i, j, precision)
         out, rd = blk.fresh_values()
-        A = blk.assemble(self._locals(x), out=out, row_defect=rd)
-        if not blk.finite():
-            require_finite_arrays([blk.E], "block (%d, %d)" % (i, j))
+        loc = self._locals(x)
+        A = blk.assemble(loc, out=out, row_defect=rd)
+        blk.check_finite(loc, "block (%d, %d)" % (i, j))
         return A
 
     def blocks(self, pairs, x, precision="fp64"):
@@ -213,6 +284,15 @@ class PDEVariationalProblem:
         todo = list(dict.fromkeys(tuple(ij) for ij in pairs))
         passes = []
         results = {}
+        # the blocks the fused route takes: one launch each, no element matrix in memory
+        for ij in list(todo):
+            b = self.block_assembly(ij[0], ij[1], precision)
+            if b.fused_plan() is not None:
+                out, rd = b.fresh_values()
+                b.assemble_values(loc, out=out, row_defect=rd)
+                results[ij] = (out, rd)
+                passes.append((b, [ij]))
+                todo.remove(ij)
         while todo:
             ij = todo.pop(0)
             blk = self.block_assembly(ij[0], ij[1], precision)
@@ -237,6 +317,8 @@ class PDEVariationalProblem:
         # block's); a set flag is traced to its block
         for blk, done in passes:
             if not blk.finite():
+                if blk.fused_plan() is not None:
+                    blk.check_finite(loc, "block (%d, %d)" % done[0])
                 for d in done:
                     require_finite_arrays([self.block_assembly(d[0], d[1], precision).E],
                                           "block (%d, %d)" % d)

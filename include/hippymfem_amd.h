@@ -236,6 +236,43 @@ int hm_scatter_vector(int64_t n, int nd, const int32_t* rptr, const int32_t* run
 int hm_csr_matvec(int64_t nrow, const int32_t* indptr, const int32_t* indices,
                   const void* data, int val_dtype, const double* x, double* y, void* stream);
 
+/* ------------------------------------------------------------------ fused assembly
+ * Element kernel and scatter in one kernel on a box mesh of nx x ny x nz hexahedra (element
+ * e = ex + nx (ey + ny ez), the problem's elem_verts and dof tables as for the other calls), so
+ * that no element matrix goes to memory.  Columns of bx x by elements are swept bottom to top,
+ * each block owning the CSR rows of its lattice points and recomputing a one-element halo on its
+ * lower sides; every CSR value is the sum of its entries in ascending element order, the order of
+ * np.bincount (hippymfem/fem/pattern.py:788), with the element-matrix kernel's bits for every
+ * entry: the values are those of hm_element_matrices followed by hm_scatter_matrix, bit for bit.
+ * The plan holds, in tile order (planes of the test lattice, then tiles x fastest, then the
+ * tile's lattice points x fastest), every row's CSR offset and row mode (0 assemble, 1 zero row,
+ * 2 zero row with diag first) and every CSR entry's place in the row's dense stencil (the box of
+ * trial-lattice points the row couples to, x fastest; bit 7 set: a column dropped by the
+ * elimination, written as 0).
+ * Replaces the assembly of a block from its element matrices (PDEVariationalProblem._block /
+ * _jacobian / setLinearizationPoint, hippymfem/modeling/PDEVariationalProblem.py:476-659,
+ * :1372-1482: QuadratureKernel.element_matrices then assemble_matrix with the folded
+ * elimination, hippymfem/fem/assemble.py:34-69, pattern.py:693-791).
+ *
+ * hm_fused_tile: the tile (bx, by) the kernel takes for block (i, j) of this problem at this
+ *   precision (HM_FP64 or HM_FP32): the layout a plan must be made for.  HM_ERR_UNSUPPORTED where
+ *   the problem has no tensor structure or the block is zero for the density.
+ * hm_assemble_fused: data (nnz) fp64 for HM_FP64, fp32 for HM_FP32, every value written. */
+typedef struct hm_fused_plan {
+  int32_t nx, ny, nz;
+  int32_t bx, by;
+  int32_t reserved;
+  const int32_t* hdr;      /* (nrow, 2): CSR offset, row mode; tile order                    */
+  const int64_t* hdr_off;  /* (planes * tiles): first row of each (plane, tile) in hdr       */
+  const uint8_t* sigma;    /* stencil place of every CSR entry, tile order, bit 7: dropped   */
+  const int64_t* sig_off;  /* (planes * tiles): start of each (plane, tile), 16-byte aligned */
+  double diag;             /* the diagonal of rows of mode 2                                 */
+} hm_fused_plan;
+
+int hm_fused_tile(const hm_problem* P, int i, int j, int precision, int32_t* bx, int32_t* by);
+int hm_assemble_fused(const hm_problem* P, int i, int j, const hm_fused_plan* F, int precision,
+                      void* data, int out_dtype, void* stream);
+
 /* ------------------------------------------------------------------ serving inputs
  * No counterpart in the reference: the seeded inputs of the serving form (assemble-and-apply
  * requests), drawn on the device from a seed held in device memory so that a captured HIP

@@ -66,7 +66,8 @@ def test_library_exports_every_declared_symbol():
 def test_ctypes_structs_match_header(tmp_path):
     """The ctypes mirrors of hm_problem and hm_scatter_plan have the header's layout: every
     field's offset and the size, as gcc lays the C structs out."""
-    structs = {"hm_problem": hm_lib.HmProblem, "hm_scatter_plan": hm_lib.HmScatterPlan}
+    structs = {"hm_problem": hm_lib.HmProblem, "hm_scatter_plan": hm_lib.HmScatterPlan,
+               "hm_fused_plan": hm_lib.HmFusedPlan}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "hippymfem_amd.h"',
              "int main(void) {"]
     for cname, cls in structs.items():
