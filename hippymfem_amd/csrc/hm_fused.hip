@@ -109,6 +109,7 @@ __global__ void __launch_bounds__(BS) k_fused(const hm_problem P, int s, int j, 
   constexpr int pt = NT - 1, pc = NC - 1;
   constexpr int NQ2 = NQ * NQ, NQ3 = NQ2 * NQ;
   constexpr int NWAVE = BS / 64;
+  static_assert(8 * FU_MAXM <= BS, "eight lanes an element slot");
   extern __shared__ double lds[];
   __shared__ int32_t UT[32];
   __shared__ int32_t TASK[FU_MAXT];
@@ -366,21 +367,26 @@ __global__ void __launch_bounds__(BS) k_fused(const hm_problem P, int s, int j, 
     const uint8_t* sg = SIG + k * Z.sig_bytes;
     const int2* hdr = HDR + k * Z.rows_max;
     for (int base = 0; base < S; base += FU_EB * BS) {
-      int kk[FU_EB], rb[FU_EB];
+      int kk[FU_EB], rb[FU_EB], rho[FU_EB];
       unsigned c[FU_EB];
       int2 hd[FU_EB];
       double v[FU_EB];
+      // every read unconditional (a place past the plane reads the last one, and is not
+      // written), so that each of the three rounds of reads is issued whole before its first use
 #pragma unroll
       for (int e = 0; e < FU_EB; ++e) {
         kk[e] = base + tid + e * BS;
-        const bool ok = kk[e] < S;
-        const int rho = ok ? ROWOF[((uint32_t)kk[e] * mul) >> 16] : 0;
-        c[e] = ok ? sg[kk[e]] : 0u;
-        hd[e] = hdr[rho];
-        rb[e] = B2D[rho] * wz;
+        const int kc = min(kk[e], S - 1);
+        rho[e] = ROWOF[((uint32_t)kc * mul) >> 16];
+        c[e] = sg[kc];
       }
 #pragma unroll
-      for (int e = 0; e < FU_EB; ++e) v[e] = kk[e] < S ? buf[rb[e] + (c[e] & 0x7F)] : 0.0;
+      for (int e = 0; e < FU_EB; ++e) {
+        hd[e] = hdr[rho[e]];
+        rb[e] = B2D[rho[e]] * wz;
+      }
+#pragma unroll
+      for (int e = 0; e < FU_EB; ++e) v[e] = buf[rb[e] + (c[e] & 0x7F)];
 #pragma unroll
       for (int e = 0; e < FU_EB; ++e)
         if (kk[e] < S) {
@@ -409,14 +415,18 @@ __global__ void __launch_bounds__(BS) k_fused(const hm_problem P, int s, int j, 
     load_planes(ez);
     // each element's alignment (every vertex coordinate that of the vertex with the same index
     // along its axis: J diagonal, the pairs of two different derivatives left out of its
-    // contraction, as the element kernel does)
-    for (int m = tid; m < M; m += BS) {
-      const double* Xe = Dreg(m);
-      bool al = true;
-      for (int v = 0; v < 8; ++v)
-        al = al && Xe[v * 3] == Xe[(v & 1) * 3] && Xe[v * 3 + 1] == Xe[(v & 2) * 3 + 1] &&
-             Xe[v * 3 + 2] == Xe[(v & 4) * 3 + 2];
-      EFLAG[m] = al ? 1 : 0;
+    // contraction, as the element kernel does; lane v of the eight lanes of element m checks
+    // vertex v, a ballot joins them)
+    {
+      const int m = tid >> 3, v = tid & 7;
+      bool ok = true;
+      if (m < M) {
+        const double* Xe = Dreg(m);
+        ok = (Xe[v * 3] == Xe[(v & 1) * 3]) & (Xe[v * 3 + 1] == Xe[(v & 2) * 3 + 1]) &
+             (Xe[v * 3 + 2] == Xe[(v & 4) * 3 + 2]);
+      }
+      const uint64_t all = __ballot(ok);
+      if (m < M && v == 0) EFLAG[m] = ((all >> (lane & 56)) & 0xFFu) == 0xFFu ? 1 : 0;
     }
     __syncthreads();
     // 2. (element, point) pairs: geometry, fields, pointwise pairs (one LDS slot for the pairs
@@ -621,7 +631,8 @@ __global__ void __launch_bounds__(BS) k_fused(const hm_problem P, int s, int j, 
         if (lz < lz_lo || lz >= lz_hi) continue;
         if (a3 < pt) {
           write_plane(lz, rk++);
-          __syncthreads();
+          // two planes of a phase lie in different buffers and slots: one barrier after both
+          if (a3 + 1 >= a3hi || lz + 1 >= lz_hi || a3 + 1 >= pt) __syncthreads();
         } else if (ez == G.nz - 1 && z1 == G.nz) {
           // the top of the box: its tables into slot 0, then written out
           rederive();

@@ -88,6 +88,8 @@ class FusedPlan:
                  col_mask=None, diag=None, device=None):
         dev = _lib.require_device() if device is None else torch.device(device)
         self.tile = bx, by = tile
+        if int(pattern.nnz) >= 2 ** 31:
+            raise NotImplementedError("CSR offsets past 2^31 (the row headers are 32-bit)")
         nx = ny = nz = int(mesh.n)
         self.dims = (nx, ny, nz)
         pt, pc = int(test_space.order), int(trial_space.order)
