@@ -17,6 +17,8 @@ FP64, FP32, BF16, FP16 = 0, 1, 2, 3
 PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16": BF16, "fp16": FP16}
 
 HM_OK = 0
+#: the largest lattice edge of ``hm_serving_field_noalloc`` (its static workspace)
+SERVING_FIELD_NOALLOC_MAX_N1 = 65
 HM_ERR_ARG, HM_ERR_HIP, HM_ERR_ROW_LONG, HM_ERR_DUP, HM_ERR_NODIAG, HM_ERR_UNSUPPORTED = (
     -1, -2, -3, -4, -5, -6)
 
@@ -26,7 +28,8 @@ EXPORTS = ("hm_last_error", "hm_version", "hm_element_matrices", "hm_element_mat
            "hm_pattern_place", "hm_pattern_repeats", "hm_pattern_work_bytes",
            "hm_pattern_build_device", "hm_pattern_place_device", "hm_scatter_matrix",
            "hm_scatter_matrix_defect", "hm_scatter_vector", "hm_csr_matvec", "hm_fused_tile",
-           "hm_assemble_fused", "hm_serving_normal", "hm_serving_field")
+           "hm_assemble_fused", "hm_serving_normal", "hm_serving_field",
+           "hm_serving_field_noalloc")
 
 P = ctypes.c_void_p
 I32, I64, DBL = ctypes.c_int32, ctypes.c_int64, ctypes.c_double
@@ -106,6 +109,7 @@ def lib():
                                              ctypes.c_int, P]),
         "hm_serving_normal": (ctypes.c_int, [P, I64, I64, P, P]),
         "hm_serving_field": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, P, P, P, P]),
+        "hm_serving_field_noalloc": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, P, P, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -13,6 +13,8 @@
 // at every draw.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "hm_common.h"
 #include "hm_device.h"
 
@@ -107,6 +109,56 @@ static unsigned grid_for(int64_t n) {
   return (unsigned)(g < 1 ? 1 : (g < 4096 ? g : 4096));
 }
 
+// the draw into a workspace of 2 n1^3 + 2 doubles: two lattice buffers and the statistics
+static int field_into(const int64_t* seed, int n1, int passes, double* work, const int64_t* to, double* out,
+                      hipStream_t st, const char* what) {
+  const int64_t n = (int64_t)n1 * n1 * n1;
+  double* a = work;
+  double* b = work + n;
+  double* stats = work + 2 * n;
+  hipLaunchKernelGGL(k_normal, dim3(grid_for(n)), dim3(256), 0, st, seed, (int64_t)0, n, a);
+  for (int p = 0; p < passes; ++p) {
+    hipLaunchKernelGGL(k_smooth, dim3(grid_for(n)), dim3(256), 0, st, (const double*)a, n1, b);
+    double* t = a;
+    a = b;
+    b = t;
+  }
+  hipLaunchKernelGGL(k_field_stats, dim3(1), dim3(FS_THREADS), 0, st, (const double*)a, n, stats);
+  hipLaunchKernelGGL(k_field_finish, dim3(grid_for(n)), dim3(256), 0, st, (const double*)a, n,
+                     (const double*)stats, to, out);
+  return hip_check(what);
+}
+
+// The workspace of hm_serving_field_noalloc: one array a device, held until the process ends.
+// Host code reaches it only through its device address (hipGetSymbolAddress, once a device).
+constexpr int FS_NOALLOC_MAX_N1 = HM_SERVING_FIELD_NOALLOC_MAX_N1;
+constexpr int64_t FS_NOALLOC_WORDS =
+    2 * (int64_t)FS_NOALLOC_MAX_N1 * FS_NOALLOC_MAX_N1 * FS_NOALLOC_MAX_N1 + 2;
+static __device__ __align__(256) double scratch[FS_NOALLOC_WORDS];
+
+constexpr int FS_MAX_DEVICES = 64;
+static std::mutex scratch_mutex;
+static double* scratch_ptr[FS_MAX_DEVICES];
+
+// the device address of the current device's workspace (looked up on its first call)
+static int scratch_of_device(double** out) {
+  int dev = 0;
+  const hipError_t g = hipGetDevice(&dev);
+  if (g != hipSuccess) HM_FAIL(HM_ERR_HIP, "hm_serving_field_noalloc: hipGetDevice: %s", hipGetErrorString(g));
+  if (dev < 0 || dev >= FS_MAX_DEVICES) HM_FAIL(HM_ERR_UNSUPPORTED, "device %d: at most %d devices", dev, FS_MAX_DEVICES);
+  std::lock_guard<std::mutex> lock(scratch_mutex);
+  if (!scratch_ptr[dev]) {
+    double* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      HM_FAIL(HM_ERR_HIP, "hm_serving_field_noalloc: the workspace's address on device %d: %s", dev,
+              hipGetErrorString(e));
+    scratch_ptr[dev] = p;
+  }
+  *out = scratch_ptr[dev];
+  return HM_OK;
+}
+
 }  // namespace hm
 
 using namespace hm;
@@ -121,20 +173,17 @@ extern "C" int hm_serving_normal(const int64_t* seed, int64_t salt, int64_t n, d
 extern "C" int hm_serving_field(const int64_t* seed, int n1, int passes, double* work, const int64_t* lattice_to_dof,
                                 double* out, void* stream) {
   HM_REQUIRE(seed && work && out && n1 >= 1 && passes >= 0, "bad field arguments");
-  const int64_t n = (int64_t)n1 * n1 * n1;
-  hipStream_t st = (hipStream_t)stream;
-  double* a = work;
-  double* b = work + n;
-  double* stats = work + 2 * n;
-  hipLaunchKernelGGL(k_normal, dim3(grid_for(n)), dim3(256), 0, st, seed, (int64_t)0, n, a);
-  for (int p = 0; p < passes; ++p) {
-    hipLaunchKernelGGL(k_smooth, dim3(grid_for(n)), dim3(256), 0, st, (const double*)a, n1, b);
-    double* t = a;
-    a = b;
-    b = t;
-  }
-  hipLaunchKernelGGL(k_field_stats, dim3(1), dim3(FS_THREADS), 0, st, (const double*)a, n, stats);
-  hipLaunchKernelGGL(k_field_finish, dim3(grid_for(n)), dim3(256), 0, st, (const double*)a, n,
-                     (const double*)stats, lattice_to_dof, out);
-  return hip_check("hm_serving_field");
+  return field_into(seed, n1, passes, work, lattice_to_dof, out, (hipStream_t)stream, "hm_serving_field");
+}
+
+extern "C" int hm_serving_field_noalloc(const int64_t* seed, int n1, int passes, const int64_t* lattice_to_dof,
+                                        double* out, void* stream) {
+  HM_REQUIRE(seed && out && n1 >= 1 && passes >= 0, "bad field arguments");
+  if (n1 > FS_NOALLOC_MAX_N1)
+    HM_FAIL(HM_ERR_ARG, "hm_serving_field_noalloc: a lattice of %d points a direction exceeds its workspace of %d",
+            n1, FS_NOALLOC_MAX_N1);
+  double* work = nullptr;
+  const int rc = scratch_of_device(&work);
+  if (rc != HM_OK) return rc;
+  return field_into(seed, n1, passes, work, lattice_to_dof, out, (hipStream_t)stream, "hm_serving_field_noalloc");
 }

@@ -288,6 +288,19 @@ int hm_serving_normal(const int64_t* seed, int64_t salt, int64_t n, double* out,
 int hm_serving_field(const int64_t* seed, int n1, int passes, double* work,
                      const int64_t* lattice_to_dof, double* out, void* stream);
 
+/* hm_serving_field_noalloc: no counterpart in the reference.  For callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops that forbid
+ * hipMalloc): hm_serving_field with its workspace in a static device array of the library
+ * (2 HM_SERVING_FIELD_NOALLOC_MAX_N1^3 + 2 doubles, about 4.4 MB, held on every device the
+ * library runs on until the process ends) instead of the caller's.  Calls no hipMalloc or
+ * hipFree; n1 above HM_SERVING_FIELD_NOALLOC_MAX_N1 gives HM_ERR_ARG.  Every call on one
+ * device uses that device's array: calls that overlap on one device (two streams, two
+ * threads) race and must be ordered by the caller.  Nothing in the array needs to start at
+ * zero (every call writes what it reads). */
+#define HM_SERVING_FIELD_NOALLOC_MAX_N1 65
+int hm_serving_field_noalloc(const int64_t* seed, int n1, int passes,
+                             const int64_t* lattice_to_dof, double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
