@@ -26,3 +26,23 @@ def assemble_vector(space, groups, element_vectors, nelem, ess=None, out=None):
     """Assemble a dual (residual) vector from per-group element vectors
     (``assemble_vector``, ``assemble.py:72-79``).  Returns a device tensor."""
     return assemble_vector_csr(space, groups, element_vectors, nelem, ess=ess, out=out)
+
+
+def assemble_scalar(comm, element_values):
+    """Sum per-element integrals into one global scalar (``assemble_scalar``,
+    ``assemble.py:82-98``): each group's values summed where they are (one device reduction
+    a group, so one scalar comes back rather than one value an element), the groups in
+    order.  ``comm`` is the reference's communicator; on one rank the local sum is the
+    global one, and ``None`` is accepted."""
+    import numpy as np
+    import torch
+
+    loc = 0.0
+    for v in element_values:
+        if isinstance(v, torch.Tensor):
+            loc += float(torch.sum(v.double()))
+        else:
+            loc += float(np.sum(np.asarray(v, dtype=np.float64)))
+    if comm is not None and getattr(comm, "Get_size", lambda: 1)() > 1:
+        raise NotImplementedError("assemble_scalar: one rank (the local sum is the global one)")
+    return loc

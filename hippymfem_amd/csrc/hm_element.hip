@@ -258,6 +258,49 @@ __global__ void __launch_bounds__(256) k_element_hvp(hm_problem P, int s, int j,
   }
 }
 
+// Element scalars R_e = sum_q w (exp(m) g(u) grad u . grad p - f p): the functional every
+// derivative above is taken of.  Lanes act as the element's points (the weighted density
+// value into the wavefront's region), then its first lane sums them in point order.
+__global__ void __launch_bounds__(256) k_element_scalars(hm_problem P, int64_t e0, int64_t e1, double* out) {
+  extern __shared__ double lds[];
+  const int nq = P.nq;
+  const WaveElement w = wave_element(P);
+  const int words = nq + 24 + 2 * P.nd_u + P.nd_m;
+  double* W = lds + (size_t)((threadIdx.x >> 6) * w.EPW + (w.in_use ? w.slot : 0)) * words;
+  double* Xe = W + nq;
+  double* ue = Xe + 24;
+  double* pe = ue + P.nd_u;
+  double* me = pe + P.nd_u;
+  const int64_t ne = e1 - e0;
+  const int64_t stride = (int64_t)gridDim.x * (blockDim.x >> 6) * w.EPW;
+  bool bad = false;
+  for (int64_t base = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * w.EPW; base < ne;
+       base += stride) {
+    const int64_t el = base + w.slot;
+    const bool act = w.in_use && el < ne;
+    if (act) load_element_lanes(P, e0 + el, w.r, Xe, ue, me, pe);
+    wave_sync();
+    if (act && w.r < nq) {
+      const int q = w.r;
+      PointState S;
+      point_state(P, q, Xe, ue, me, pe, S);
+      const double* Nu = P.tab_u + (size_t)q * P.nd_u;
+      double pq = 0;
+      for (int a = 0; a < P.nd_u; ++a) pq = fma(Nu[a], pe[a], pq);
+      W[q] = S.w * (S.kap * S.g * S.dot - P.f * pq);
+    }
+    wave_sync();
+    if (act && w.r == 0) {
+      double acc = 0;
+      for (int q = 0; q < nq; ++q) acc += W[q];
+      out[el] = acc;
+      bad |= !isfinite(acc);
+    }
+    wave_sync();
+  }
+  if (bad && P.nonfinite) atomicOr(P.nonfinite, 1);
+}
+
 static int check_problem(const hm_problem* P) {
   HM_REQUIRE(P != nullptr, "no problem given");
   HM_REQUIRE(P->ne >= 0 && P->nq > 0 && P->nq <= 64, "quadrature points per element %d outside 1..64", P->nq);
@@ -457,6 +500,19 @@ extern "C" int hm_element_vectors(const hm_problem* P, int i, int64_t e0, int64_
   hipLaunchKernelGGL(k_element_vectors, dim3(grid_for((e1 - e0 + epb - 1) / epb)), dim3(256), bytes,
                      (hipStream_t)stream, *P, i, e0, e1, out);
   return hip_check("hm_element_vectors");
+}
+
+extern "C" int hm_element_scalars(const hm_problem* P, int64_t e0, int64_t e1, double* out, void* stream) {
+  if (int rc = check_problem(P)) return rc;
+  HM_REQUIRE(e0 >= 0 && e0 <= e1 && e1 <= P->ne, "elements outside the batch");
+  HM_REQUIRE(e1 == e0 || out, "no output given");
+  if (e1 == e0) return HM_OK;
+  const int epb = 4 * wave_elements(*P);
+  const size_t bytes = (size_t)epb * ((size_t)P->nq + 24 + 2 * P->nd_u + P->nd_m) * 8;
+  if (int rc = fit_lds((const void*)k_element_scalars, bytes)) return rc;
+  hipLaunchKernelGGL(k_element_scalars, dim3(grid_for((e1 - e0 + epb - 1) / epb)), dim3(256), bytes,
+                     (hipStream_t)stream, *P, e0, e1, out);
+  return hip_check("hm_element_scalars");
 }
 
 extern "C" int hm_element_hvp(const hm_problem* P, int i, int j, const double* dir, int64_t e0,

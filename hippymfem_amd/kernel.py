@@ -234,6 +234,24 @@ class QuadratureKernel:
         self._hold(keep, stream)
         return [out]
 
+    def element_values(self, local_arrays, params=(), elements=None, out=None, stream=None):
+        """Per-element integrals of the density ``[(ne,)]`` (``element_values``,
+        ``kernel.py:2129-2136``; the functional of ``_functional``, ``kernel.py:977-1009``):
+        ``sum_q w_q |det J| (exp(m) g(u) grad u . grad p - f p)``, summed in point order.
+        ``RuntimeError`` where one is not finite."""
+        flag = torch.zeros(1, dtype=torch.int32, device=_lib.require_device())
+        P, keep = self.problem(local_arrays, flag)
+        a, b = (0, self.group.ne) if elements is None else elements
+        if out is None:
+            out = torch.empty(b - a, dtype=torch.float64, device=keep[0].device)
+        _lib.check(_lib.lib().hm_element_scalars(_lib.ctypes.byref(P), int(a), int(b),
+                                                 _lib.ptr(out), _lib.stream_handle(stream)),
+                   "element values")
+        self._hold(keep + (flag,), stream)
+        if int(flag.item()) != 0:
+            raise RuntimeError("element values: a value is not finite")
+        return [out]
+
     def element_hvp(self, i, j, local_arrays, jdir_local, params=(), elements=None, out=None,
                     stream=None):
         """Block ``(i, j)`` applied to the slot-``j`` direction element by element

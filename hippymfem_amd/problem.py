@@ -7,6 +7,8 @@ request, a benchmark step) launches kernels into resident memory and allocates n
 """
 
 import contextlib
+import os
+import warnings
 
 import numpy as np
 import torch
@@ -61,6 +63,32 @@ def require_finite(v, what):
     return v
 
 
+#: Bytes an element batch's resident matrices may take on the device (``None``: what the
+#: device has free).  A batch above it, or one whose allocation fails, is assembled in
+#: chunks (``BlockAssembly``); ``HM_ELEMENT_BUDGET`` in the environment sets it too.
+ELEMENT_BUDGET = None
+
+#: How many element batches of this process did not fit and were assembled in chunks (the
+#: reference's ``RETRIES``, ``kernel.py:608-612``): 0 where every batch fit.
+RETRIES = 0
+
+
+def _budget():
+    if ELEMENT_BUDGET is not None:
+        return int(ELEMENT_BUDGET)
+    env = os.environ.get("HM_ELEMENT_BUDGET")
+    return int(float(env)) if env else None
+
+
+def _fits(ne, want, free):
+    """Elements of a chunk, from the size the whole batch asked for: a third of what is free
+    (the failed request is the largest buffer, not the whole working set), at least halved
+    (``_fits``, ``kernel.py:629-656``)."""
+    if free <= 0 or want <= 0:
+        return max(1, ne // 2)
+    return max(1, min(ne // 2, int(ne * (0.33 * free) / want)))
+
+
 def _symmetric(i, j):
     """Whether block (i, j) is eliminated on both sides where they share a space
     (``PDEVariationalProblem._symmetric``)."""
@@ -100,6 +128,7 @@ class BlockAssembly:
             value_dtype = e_dtype
         self.e_dtype = e_dtype
         self._E = None
+        self.chunk = None   # elements a chunk when the whole batch does not fit
         self.data = torch.empty(self.pattern.nnz, dtype=value_dtype, device=dev)
         self.row_defect = None
         if row_sums and Vi is Vj and value_dtype != torch.float64:
@@ -109,12 +138,47 @@ class BlockAssembly:
 
     @property
     def E(self):
-        """The resident element matrices ``(ne, nd_i, nd_j)``, made at the first use."""
+        """The resident element matrices ``(ne, nd_i, nd_j)``, made at the first use; where
+        the whole batch does not fit (the allocation fails, or it is above
+        ``ELEMENT_BUDGET``), a buffer of one chunk (``chunk`` elements) and a warning: the
+        block is then assembled chunk by chunk, every chunk's entries added to the values in
+        order (the reference's retry in chunks sized from the failed request,
+        ``kernel.py:1080-1113``)."""
         if self._E is None:
+            global RETRIES
             k = self.kernel
-            self._E = torch.empty((k.group.ne, k.nd[self.i], k.nd[self.j]), dtype=self.e_dtype,
-                                  device=self.data.device)
+            ne, row = k.group.ne, k.nd[self.i] * k.nd[self.j]
+            size = torch.empty((), dtype=self.e_dtype).element_size()
+            want = ne * row * size
+            budget = _budget()
+            try:
+                if budget is not None and want > budget:
+                    raise torch.cuda.OutOfMemoryError(
+                        "allocate %d bytes: above the element budget of %d" % (want, budget))
+                self._E = torch.empty((ne, k.nd[self.i], k.nd[self.j]), dtype=self.e_dtype,
+                                      device=self.data.device)
+            except torch.cuda.OutOfMemoryError:
+                RETRIES += 1
+                free = budget if budget is not None else torch.cuda.mem_get_info(
+                    self.data.device)[0]
+                self.chunk = _fits(ne, want, free)
+                warnings.warn(
+                    "element batch of %d (%.3g GB of element matrices) did not fit on %s; "
+                    "assembling block (%d, %d) in chunks of %d. FP64 values are the same bits "
+                    "as a whole-batch assembly; single-precision values differ at round-off. "
+                    "Set ELEMENT_BUDGET (HM_ELEMENT_BUDGET) to pin the budget."
+                    % (ne, want / 2 ** 30, self.data.device, self.i, self.j, self.chunk),
+                    RuntimeWarning, stacklevel=2)
+                self._E = torch.empty((self.chunk, k.nd[self.i], k.nd[self.j]),
+                                      dtype=self.e_dtype, device=self.data.device)
         return self._E
+
+    @property
+    def streamed(self):
+        """Whether the two-pass route of this block runs chunk by chunk (its batch did not
+        fit)."""
+        self.E
+        return self.chunk is not None
 
     def fused_plan(self):
         """The plan of the fused route (``fused.FusedPlan``), made at the first use, or None
@@ -164,9 +228,39 @@ class BlockAssembly:
             self.nonfinite.zero_()
 
     def element_matrices(self, local_arrays, stream=None):
+        if self.streamed:
+            raise MemoryError("block (%d, %d): the element batch does not fit; it is assembled "
+                              "in chunks of %d (assemble_values)" % (self.i, self.j, self.chunk))
         self.clear_flag(stream)
         return self.kernel.element_matrices(self.i, self.j, local_arrays, precision=self.precision,
                                             out=self.E, stream=stream, nonfinite=self.nonfinite)[0]
+
+    def chunks(self):
+        """The ``[a, b)`` element ranges of a streamed assembly."""
+        ne = self.kernel.group.ne
+        return [(a, min(a + self.chunk, ne)) for a in range(0, ne, self.chunk)]
+
+    def _assemble_streamed(self, local_arrays, stream, out, row_defect):
+        """Chunk after chunk: its element matrices into the chunk buffer, then added to the
+        values (the scatter's rows take the chunk's entries after the ones already summed, so
+        the FP64 sums are those of the whole batch)."""
+        self.clear_flag(stream)
+        if out is None:
+            out, row_defect = self.data, self.row_defect
+        if row_defect is not None:
+            raise NotImplementedError("row sums of a block assembled in chunks")
+        k = self.kernel
+        p = self.dplan
+        for n, (a, b) in enumerate(self.chunks()):
+            E = self.E[:b - a]
+            k.element_matrices(self.i, self.j, local_arrays, precision=self.precision,
+                               elements=(a, b), out=E, stream=stream, nonfinite=self.nonfinite)
+            _lib.check(_lib.lib().hm_scatter_matrix(
+                _lib.ctypes.byref(p["plan"]), _lib.ptr(E), _lib.FP64 if E.dtype == torch.float64
+                else _lib.FP32, int(a), int(b), _lib.ptr(out),
+                _lib.FP64 if out.dtype == torch.float64 else _lib.FP32, 0 if n == 0 else 1,
+                _lib.stream_handle(stream)), "scatter of elements [%d, %d)" % (a, b))
+        return out
 
     def finite(self):
         """Whether the last element pass wrote finite values only (one int32 read back)."""
@@ -198,6 +292,8 @@ class BlockAssembly:
             return F.assemble(self.kernel, self.i, self.j, local_arrays, self.precision,
                               self.data if out is None else out, stream=stream,
                               nonfinite=self.nonfinite)
+        if self.streamed:
+            return self._assemble_streamed(local_arrays, stream, out, row_defect)
         self.element_matrices(local_arrays, stream)
         return self.scatter(stream, out=out, row_defect=row_defect)
 
@@ -213,6 +309,13 @@ class BlockAssembly:
         (``require_finite_arrays``); the element matrices are made for the check when the
         fused route left none."""
         if self.finite():
+            return
+        if self.streamed:
+            for a, b in self.chunks():
+                E = self.E[:b - a]
+                self.kernel.element_matrices(self.i, self.j, local_arrays,
+                                             precision=self.precision, elements=(a, b), out=E)
+                require_finite_arrays([E], what)
             return
         if self.fused_plan() is not None:
             self.element_matrices(local_arrays)
@@ -284,10 +387,11 @@ class PDEVariationalProblem:
         todo = list(dict.fromkeys(tuple(ij) for ij in pairs))
         passes = []
         results = {}
-        # the blocks the fused route takes: one launch each, no element matrix in memory
+        # the blocks the fused route takes (one launch each, no element matrix in memory) and
+        # the blocks whose element batch does not fit (a chunked pass each)
         for ij in list(todo):
             b = self.block_assembly(ij[0], ij[1], precision)
-            if b.fused_plan() is not None:
+            if b.fused_plan() is not None or b.streamed:
                 out, rd = b.fresh_values()
                 b.assemble_values(loc, out=out, row_defect=rd)
                 results[ij] = (out, rd)
@@ -317,8 +421,9 @@ class PDEVariationalProblem:
         # block's); a set flag is traced to its block
         for blk, done in passes:
             if not blk.finite():
-                if blk.fused_plan() is not None:
+                if blk.fused_plan() is not None or blk.streamed:
                     blk.check_finite(loc, "block (%d, %d)" % done[0])
+                    continue
                 for d in done:
                     require_finite_arrays([self.block_assembly(d[0], d[1], precision).E],
                                           "block (%d, %d)" % d)
@@ -347,6 +452,14 @@ class PDEVariationalProblem:
                 need.append((STATE, STATE))
             need += [(STATE, PARAMETER), (PARAMETER, PARAMETER)]
         return self.blocks(need, x, precision)
+
+    def functional(self, x):
+        """The integral of the residual density itself, ``R(u, m, p)`` (``functional``,
+        ``PDEVariationalProblem.py:1802-1817``, domain part): the element values summed
+        (``assemble_scalar``)."""
+        from .assemble import assemble_scalar
+
+        return assemble_scalar(None, self.kernel.element_values(self._locals(x)))
 
     def apply_ij_at(self, i, j, x, dir, out=None):
         """Block ``(i, j)`` at ``x`` applied to ``dir`` without assembling it

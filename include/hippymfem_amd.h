@@ -124,6 +124,13 @@ int hm_consistent_block(const float* K, const double* Rj, const double* Ri, int6
 int hm_element_vectors(const hm_problem* P, int i, int64_t e0, int64_t e1, double* out,
                        void* stream);
 
+/* Element scalars R_e = sum_q w_q |det J| (exp(m) g(u) grad u . grad p - f p) for [e0, e1):
+ * out (e1-e0) fp64, each the sum over its points in point order; sets *nonfinite where a value
+ * is not finite.  Replaces GroupKernel.residual_value and QuadratureKernel.element_values
+ * (hippymfem/fem/kernel.py:1859-1865, :2129-2136) on the functional of
+ * GroupKernel._functional (kernel.py:977-1009). */
+int hm_element_scalars(const hm_problem* P, int64_t e0, int64_t e1, double* out, void* stream);
+
 /* Block (i, j) applied element by element to the slot-j dof vector dir:
  * out (e1-e0, nd_i) fp64.  Replaces GroupKernel.hvp_block (hippymfem/fem/kernel.py:1756-1782). */
 int hm_element_hvp(const hm_problem* P, int i, int j, const double* dir, int64_t e0,

@@ -448,6 +448,16 @@ def element_matrices(data, density, i, j, u, m, p, elems=slice(None)):
                      Nc, physical_gradients(Gc, Jinv))
 
 
+def element_values(data, density, u, m, p, elems=slice(None)):
+    """Element scalars ``R_e = sum_q w |det J| (exp(m) g(u) grad u . grad p - f p)``: the
+    functional every derivative is taken of (``_functional``, ``kernel.py:977-1009``;
+    ``residual_value``/``element_values``, ``kernel.py:1859-1865``, ``:2129-2136``)."""
+    uv, ug, mv, pv, pg = data.fields(u, m, p, elems)
+    g, _, _ = density.g(uv)
+    r = np.exp(mv) * g * np.einsum("eqd,eqd->eq", ug, pg) - density.f * pv
+    return np.sum(data.wdet[elems] * r, axis=1)
+
+
 def element_vectors(data, density, i, u, m, p, elems=slice(None)):
     """``dR_e / d(slot i dofs)`` (``grad_block``, ``kernel.py:1343-1355``), closed form."""
     uv, ug, mv, pv, pg = data.fields(u, m, p, elems)
@@ -658,6 +668,9 @@ class Problem:
 
     def vector(self, i, elems=slice(None)):
         return element_vectors(self.data, self.density, i, self.u, self.m, self.p, elems)
+
+    def values(self, elems=slice(None)):
+        return element_values(self.data, self.density, self.u, self.m, self.p, elems)
 
     #: elimination of each block (``PDEVariationalProblem.py:1448-1451``, ``:597-598``)
     ELIMINATION = {(ADJOINT, STATE): dict(test=True, diag_policy="one"),

@@ -153,11 +153,28 @@ def test_brute_force_extended_precision():
 
 
 def _functional(data, density, u, m, p):
-    """R = sum_e sum_q w |det J| r(u, m, p) written from the density itself."""
-    uv, ug, mv, pv, pg = data.fields(u, m, p)
-    g = 1.0 + uv ** 2 if density.kind == "nonlinear" else 1.0
-    r = np.exp(mv) * g * np.einsum("eqd,eqd->eq", ug, pg) - density.f * pv
-    return float(np.sum(data.wdet * r))
+    """R = sum_e R_e, the oracle's element values summed."""
+    return float(np.sum(O.element_values(data, density, u, m, p)))
+
+
+@pytest.mark.parametrize("perturb", [0.0, 0.2])
+@pytest.mark.parametrize("kind", ["linear", "nonlinear"])
+def test_element_values_closed_forms(kind, perturb):
+    """On the unit cube with m = 0 (kappa = 1), f = 1: u = p = x gives
+    int (g(x) - x) dx = 1/2 (g = 1) or 5/6 (g = 1 + x^2); u = 0, p = 1 gives -1 (the volume).
+    x is in the Q2 space of the trilinear elements and the integrands have degree <= 4 a
+    direction with det J, so the 3- and 4-point rules are exact, perturbed or not."""
+    for n1d in (3, 4):
+        data, Vu, Vm = _data(2, perturb=perturb, n1d=n1d)
+        dens = O.Density(kind)
+        x = Vu.dof_coordinates()[:, 0]
+        m0 = np.zeros(Vm.ndof)
+        want = 0.5 if kind == "linear" else 5.0 / 6.0
+        got = float(np.sum(O.element_values(data, dens, x, m0, x)))
+        assert abs(got - want) <= 1e-13, (n1d, got)
+        vol = float(np.sum(O.element_values(data, dens, np.zeros(Vu.ndof), m0,
+                                            np.ones(Vu.ndof))))
+        assert abs(vol + 1.0) <= 1e-13, (n1d, vol)
 
 
 @pytest.mark.parametrize("n1d", [3, 4])
