@@ -327,6 +327,40 @@ def default_rule(args, loc, stream):
     return out
 
 
+def block_setup(args):
+    """One-time set-up of the headline block on fresh spaces (nothing cached): the CSR
+    pattern built on the device (``hm_pattern_build_device``, ``hm_pattern_place_device``),
+    the elimination plan, then the fused kernel's plan (row headers and stencil maps), and the
+    two-pass route's scatter plan (codes in run order, row headers) for comparison.  Host wall
+    time, synchronized."""
+    import torch
+
+    from hippymfem_amd import ADJOINT, STATE
+    from hippymfem_amd.problem import PDEVariationalProblem
+    from hippymfem_amd.workloads import Workload
+
+    W = Workload(args.n)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    prob = PDEVariationalProblem(W.spaces, W.density, W.batches, W.bc)
+    blk = prob.block_assembly(ADJOINT, STATE)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    fused = blk.fused_plan() is not None
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    blk.dplan
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    out = {"block_setup_ms": 1e3 * (t2 - t0), "pattern_ms": 1e3 * (t1 - t0),
+           "fused_plan_ms": 1e3 * (t2 - t1) if fused else None,
+           "two_pass_scatter_plan_ms": 1e3 * (t3 - t2),
+           "route": "fused" if fused else "two-pass"}
+    del blk, prob, W
+    torch.cuda.empty_cache()
+    return out
+
+
 def copy_calibration():
     import torch
 
@@ -428,6 +462,7 @@ def main():
         out["default_rule"] = default_rule(args, loc, stream)
     out["fp32_form"] = fp32_form(prob, loc, blk, W, args, stream)
     out["fp32_form"]["speedup_vs_fp64"] = step / (out["fp32_form"]["ms_per_step"] * 1e-3)
+    out["setup"] = block_setup(args)
     if not args.no_serving:
         from hippymfem_amd import serving
 

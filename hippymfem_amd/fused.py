@@ -64,6 +64,14 @@ def _zeros(kernel, s):
     return np.zeros(1)
 
 
+def _pattern_array(pattern, name, dev):
+    """The pattern's ``indptr``/``indices`` on ``dev``: its device arrays on the GPU (no host
+    round trip), its host arrays for a plan built on the CPU."""
+    if dev.type == "cuda":
+        return pattern.dev(name).to(dev)
+    return torch.as_tensor(np.asarray(getattr(pattern, name)))
+
+
 def _widths(L, p, n, pc):
     """Stencil width of every lattice point ``0..L-1`` of order ``p`` on ``n`` elements."""
     l = np.arange(L)
@@ -126,7 +134,7 @@ class FusedPlan:
         l2d_t = torch.as_tensor(np.asarray(test_space.lattice_to_dof), dtype=torch.int64,
                                 device=dev)
         r = l2d_t[rows_lat]
-        indptr = torch.as_tensor(np.asarray(pattern.indptr), dtype=torch.int64, device=dev)
+        indptr = _pattern_array(pattern, "indptr", dev).long()
         lo = indptr[r]
         ln = indptr[r + 1] - lo
         lxr, lyr, lzr = rows_lat % L, (rows_lat // L) % L, rows_lat // (L * L)
@@ -146,7 +154,7 @@ class FusedPlan:
             raise NotImplementedError("the tiles do not cover the pattern")
         starts = torch.cumsum(ln, 0) - ln
         idx = torch.repeat_interleave(lo - starts, ln) + torch.arange(total, device=dev)
-        cols = torch.as_tensor(np.asarray(pattern.indices), device=dev)[idx].long()
+        cols = _pattern_array(pattern, "indices", dev)[idx].long()
         d2l_c = torch.empty(C ** 3, dtype=torch.int64, device=dev)
         d2l_c[torch.as_tensor(np.asarray(trial_space.lattice_to_dof), dtype=torch.int64,
                               device=dev)] = torch.arange(C ** 3, device=dev)

@@ -18,8 +18,8 @@ class ParCSRMatrix:
     def __init__(self, pattern, data, row_starts=None, col_starts=None):
         self.pattern = pattern
         dev = data.device
-        base = pattern.device_plan()["keep"][0] if dev.type == "cuda" else None
-        self.indptr = base["indptr"] if base is not None else torch.as_tensor(pattern.indptr)
+        self.indptr = (pattern.dev("indptr") if dev.type == "cuda"
+                       else torch.as_tensor(pattern.indptr))
         self._indices = None
         self.data = data
         self.shape = (pattern.nrow, pattern.ncol)
@@ -32,7 +32,8 @@ class ParCSRMatrix:
     @property
     def indices(self):
         if self._indices is None:
-            self._indices = torch.as_tensor(self.pattern.indices).to(self.data.device)
+            self._indices = (self.pattern.dev("indices") if self.data.device.type == "cuda"
+                             else torch.as_tensor(self.pattern.indices))
         return self._indices
 
     @property

@@ -150,35 +150,76 @@ class ScatterPattern:
         self._masked = {}
 
     def _build_host(self, nthreads):
-        (indptr, self.rptr, self.runs, self.pos, self.nnz,
+        (indptr, rptr, runs, pos, self.nnz,
          self.max_row) = build_tables(self.tdofs, self.cdofs, self.nrow, self.ncol,
                                       self.diag_first, nthreads)
         if self.nnz > np.iinfo(np.int32).max:
             raise ValueError("the sparsity pattern has %d entries, beyond int32" % self.nnz)
-        self._indptr64 = indptr
-        self.indptr = indptr.astype(np.int32)
-        self.indices = np.empty(self.nnz, _HYPRE_INT)
+        indices = np.empty(self.nnz, _HYPRE_INT)
         L = _lib.lib()
         _lib.check(L.hm_pattern_place(_lib.ptr(self.cdofs), self.ne, self.nd_t, self.nd_c,
-                                      self.nrow, _lib.ptr(self.rptr), _lib.ptr(self.runs),
-                                      _lib.ptr(indptr), _lib.ptr(self.pos), _threads(),
-                                      _lib.ptr(self.indices), None), "pattern place")
+                                      self.nrow, _lib.ptr(rptr), _lib.ptr(runs),
+                                      _lib.ptr(indptr), _lib.ptr(pos), _threads(),
+                                      _lib.ptr(indices), None), "pattern place")
+        self._h = dict(indptr=indptr.astype(np.int32), indptr64=indptr, rptr=rptr, runs=runs,
+                       pos=pos, indices=indices)
+        self._d = {}
         rep = _lib.I32(0)
-        _lib.check(L.hm_pattern_repeats(self.ne, self.nd_t, self.nd_c, _lib.ptr(self.pos),
+        _lib.check(L.hm_pattern_repeats(self.ne, self.nd_t, self.nd_c, _lib.ptr(pos),
                                         _threads(), _lib.ctypes.byref(rep)), "pattern check")
         return bool(rep.value)
 
     def _build_device(self):
         d = build_tables_device(self.tdofs, self.cdofs, self.nrow, self.ncol, self.diag_first)
         self.nnz, self.max_row = d["nnz"], d["max_row"]
-        self.indptr = d["indptr"].cpu().numpy()
-        self._indptr64 = self.indptr.astype(np.int64)
-        self.rptr = d["rptr"].cpu().numpy().astype(np.int64)
-        self.runs = d["runs"].cpu().numpy()
-        self.pos = d["pos"].cpu().numpy().view(np.uint16)
-        self.indices = d["indices"].cpu().numpy()
-        self._plans["base"] = dict(indptr=d["indptr"], rptr=d["rptr"], runs=d["runs"])
+        self._d = {k: d[k] for k in ("indptr", "rptr", "runs", "pos", "indices")}
+        self._h = {}
         return d["repeats"]
+
+    # ------------------------------------------------------- the arrays, either side
+    def dev(self, name):
+        """A device tensor of the pattern: ``indptr``, ``rptr``, ``runs``, ``indices`` (int32)
+        or ``pos`` (uint16 words held as int16).  A device build keeps its own; a host build's
+        are copied once."""
+        got = self._d.get(name)
+        if got is None:
+            h = self._h[name]
+            if name == "rptr":
+                h = h.astype(np.int32)
+            elif name == "pos":
+                h = h.view(np.int16)
+            got = self._d[name] = torch.as_tensor(np.ascontiguousarray(h)).to(
+                _lib.require_device())
+        return got
+
+    def _host(self, name):
+        got = self._h.get(name)
+        if got is None:
+            if name == "indptr64":
+                got = self._host("indptr").astype(np.int64)
+            else:
+                got = self._d[name].cpu().numpy()
+                if name == "rptr":
+                    got = got.astype(np.int64)
+                elif name == "pos":
+                    got = got.view(np.uint16)
+            self._h[name] = got
+        return got
+
+    #: host arrays (numpy), copied from the device build when first asked for
+    indptr = property(lambda self: self._host("indptr"))
+    indices = property(lambda self: self._host("indices"))
+    rptr = property(lambda self: self._host("rptr"))
+    runs = property(lambda self: self._host("runs"))
+    pos = property(lambda self: self._host("pos"))
+    _indptr64 = property(lambda self: self._host("indptr64"))
+
+    def dev_cdofs(self):
+        """The trial dof table ``(ne, nd_c)`` int32 on the device."""
+        got = self._d.get("cdofs")
+        if got is None:
+            got = self._d["cdofs"] = torch.as_tensor(self.cdofs).to(_lib.require_device())
+        return got
 
     # ---------------------------------------------------------------- host arrays
     @property
@@ -222,42 +263,37 @@ class ScatterPattern:
         if got is not None and got["row_mask"] is row_mask and got["col_mask"] is col_mask:
             return got
         dev = _lib.require_device()
-        base = self._plans.get("base")
-        if base is None:
-            base = self._plans["base"] = dict(
-                indptr=torch.as_tensor(self.indptr).to(dev),
-                rptr=torch.as_tensor(self.rptr.astype(np.int32)).to(dev),
-                runs=torch.as_tensor(self.runs).to(dev))
+        base = dict(indptr=self.dev("indptr"), rptr=self.dev("rptr"), runs=self.dev("runs"))
         wide = self.max_row > 254
-        code_t = np.uint16 if wide else np.uint8
-        drop = np.iinfo(code_t).max
         codes_key = ("codes", id(col_mask))
         codes = self._plans.get(codes_key)
         if codes is None or codes[0] is not col_mask:
-            c = self.pos.astype(code_t)
+            # on the device: the place codes (uint8 for rows below 255 entries, else the
+            # uint16 words as int16), the dropped columns all ones, in run order (the codes
+            # of the run at position p at p nd_c, a row's together)
+            pos = self.dev("pos")
+            c = pos.clone() if wide else pos.to(torch.uint8)
             if col_mask is not None:
-                cm = np.asarray(col_mask, dtype=bool)[self.cdofs]            # (ne, nd_c)
-                dropped = np.broadcast_to(cm[:, None, :], (self.ne, self.nd_t, self.nd_c))
-                c = c.reshape(self.ne, self.nd_t, self.nd_c)
-                c[dropped] = drop
-                c = c.reshape(-1)
-            # in run order: the codes of the run at position p at p * nd_c, a row's together
-            c = np.ascontiguousarray(c.reshape(-1, self.nd_c)[self.runs]).reshape(-1)
-            ct = torch.from_numpy(c.view(np.uint8) if not wide else c.view(np.int16)).to(dev)
-            codes = self._plans[codes_key] = (col_mask, ct)
+                cm = torch.as_tensor(np.asarray(col_mask, dtype=bool)).to(dev)
+                cm = cm[self.dev_cdofs().long()]                                 # (ne, nd_c)
+                c = c.view(self.ne, self.nd_t, self.nd_c)
+                c.masked_fill_(cm[:, None, :].expand(self.ne, self.nd_t, self.nd_c),
+                               -1 if wide else 255)
+            c = c.view(-1, self.nd_c)[base["runs"].long()].reshape(-1).contiguous()
+            codes = self._plans[codes_key] = (col_mask, c)
         rowmode = None
         if row_mask is not None:
-            rm = np.asarray(row_mask, dtype=bool)
+            rm = torch.as_tensor(np.asarray(row_mask, dtype=bool)).to(dev)
             if diag is not None:
                 if not self.diag_first:
                     raise ValueError("a diagonal is written on square patterns only")
-                rows = np.flatnonzero(rm)
-                first = self.indptr[rows]
-                if rows.size and not np.array_equal(self.indices[first], rows):
+                rows = torch.nonzero(rm).reshape(-1)
+                first = base["indptr"].long()[rows]
+                bad = int((self.dev("indices")[first].long() != rows).sum()) if rows.numel() else 0
+                if bad:
                     raise RuntimeError("%d of %d eliminated rows have no diagonal entry"
-                                       % (int(np.count_nonzero(self.indices[first] != rows)),
-                                          rows.size))
-            rowmode = torch.from_numpy((rm.astype(np.int8) * (2 if diag is not None else 1))).to(dev)
+                                       % (bad, rows.numel()))
+            rowmode = rm.to(torch.int8) * (2 if diag is not None else 1)
         head = self._row_headers(base, rowmode) if self.nd_c > 8 else None
         plan = _lib.HmScatterPlan(
             nrow=self.nrow, nnz=self.nnz, ne=self.ne, nd_t=self.nd_t, nd_c=self.nd_c,

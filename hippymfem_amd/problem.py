@@ -120,7 +120,7 @@ class BlockAssembly:
         self.plan = plan_block(Vi, Vj, kernel.batches.groups, test_ess=test_ess,
                                diag_policy=diag_policy, symmetric=_symmetric(i, j))
         self.pattern = self.plan.pattern
-        self.dplan = _device_plan(self.plan)
+        self._dplan = None
         self.precision = precision
         dev = _lib.require_device()
         e_dtype = torch.float64 if precision == "fp64" else torch.float32
@@ -174,6 +174,14 @@ class BlockAssembly:
         return self._E
 
     @property
+    def dplan(self):
+        """The scatter's device plan (codes in run order, row modes), made at the first
+        two-pass assembly: the fused route does not read it."""
+        if self._dplan is None:
+            self._dplan = _device_plan(self.plan)
+        return self._dplan
+
+    @property
     def streamed(self):
         """Whether the two-pass route of this block runs chunk by chunk (its batch did not
         fit)."""
@@ -194,6 +202,12 @@ class BlockAssembly:
                 why = "values of another precision than the element matrices"
             elif self.plan.diag_policy == "keep" and self.plan.diag is not None:
                 why = "diag_policy 'keep'"
+            elif self.route == "auto" and self.kernel.spaces[self.j].order < 2:
+                # a Q1 trial space: the element matrices are small against the point work
+                # the fused kernel repeats for its halo, and the two-pass route takes two such
+                # blocks from one element pass (64^3 P2: C 1.04 ms fused against 0.50 ms a
+                # block for C and W_um together, W_mm 0.78 against 0.36 ms)
+                why = "a Q1 trial space (the two-pass route is faster)"
             else:
                 why = fused.supported(self.kernel, self.i, self.j, self.precision)
             tile = None

@@ -140,8 +140,8 @@ class Model:
         self.ndof = W.Vu.ndof
         self.ndof_m = W.Vm.ndof
         # CSR index arrays on the device (the matrix-vector product reads them)
-        self.indptr = torch.as_tensor(self.pattern.indptr).to(device)
-        self.indices = torch.as_tensor(self.pattern.indices).to(device)
+        self.indptr = self.pattern.dev("indptr").to(device)
+        self.indices = self.pattern.dev("indices").to(device)
         # lattice order -> dof order of the parameter space, and the fields A does not read
         self.lattice_to_dof_m = torch.as_tensor(W.Vm.lattice_to_dof.astype(np.int64)).to(device)
         self.zeros_u = torch.zeros(self.ndof, dtype=torch.float64, device=device)

@@ -1,5 +1,5 @@
-"""Every block at its full size, through ``PDEVariationalProblem`` (the fused route on these box
-meshes): 64^3 linear (C, W_um, W_mm against the oracle on an element sample), 96^3 nonlinear
+"""Every block at its full size, through ``PDEVariationalProblem`` on the fused route (the kernel
+of these box meshes): 64^3 linear (C, W_um, W_mm against the oracle on an element sample), 96^3 nonlinear
 and 128^3 perturbed (size-independent properties).
 
 Properties that fix a mistake the oracle comparison at small sizes could miss at these sizes:
@@ -89,7 +89,7 @@ def test_64cube_hessian_blocks(gpu):
     and 512 elements against the oracle."""
     n = 64
     W = Workload(n)
-    fu = problem(W, "auto")
+    fu = problem(W, "fused")
     tp = problem(W, "two-pass")
     P = O.Problem(n)
     elems = np.sort(np.random.default_rng(3).choice(W.mesh.ne, 512, replace=False))
@@ -111,7 +111,7 @@ def test_64cube_hessian_blocks(gpu):
 
 def _properties(W, gpu, blocks_expected):
     n = W.n
-    prob = problem(W)
+    prob = problem(W, "fused")
     got = prob.set_linearization_point(point(W))
     assert sorted(got) == sorted(blocks_expected)
     got32 = prob.set_linearization_point(point(W), precision="fp32")
