@@ -145,3 +145,43 @@ def test_graph_replays_equal_eager(gpu):
             assert torch.equal(r.y, ref[s]), (n, order, s)
             device_field(n, 1000 + k, gpu)            # eager work between the replays
             torch.cuda.synchronize()
+
+
+@pytest.mark.gpu
+def test_batched_requests_equal_alone(gpu):
+    """Requests of one small problem arriving together are served by batched launches (a
+    model of k disjoint copies of the mesh); each result is the bits of the request alone."""
+    import torch
+
+    from hippymfem_amd.serving import ModelCache, Request, ServingEngine
+
+    cache = ModelCache("fp32")
+    cache.warm(sizes=(8, 16), buffers=2)
+    eng = ServingEngine(cache, slots=2)
+    reqs = [Request(k, 8 if k % 3 else 16, 1 + k % 2, 500 + k, arrival=0.0) for k in range(14)]
+    eng.run(reqs)
+    assert max(r.batch for r in reqs) > 1
+    assert eng.launches < len(reqs)
+    single = ModelCache("fp32", batching=False)
+    for r in reqs:
+        alone = Request(100 + r.rid, r.n, r.order, r.seed)
+        ServingEngine(single, slots=1).run([alone])
+        assert torch.equal(alone.y, r.y), "request %d (batch of %d) differs alone" % (r.rid,
+                                                                                       r.batch)
+
+
+@pytest.mark.gpu
+def test_small_requests_pass_a_large_one(gpu):
+    """Size-aware admission: with the large problem's slots held, small requests that arrive
+    after a large one are admitted before it."""
+    from hippymfem_amd.serving import ModelCache, Request, ServingEngine
+
+    cache = ModelCache("fp32")
+    cache.warm(sizes=(8, 48), orders=(2,), buffers=2)
+    assert cache.get(48, 2).cost > cache.get(8, 2).cost
+    eng = ServingEngine(cache, slots=4, large=0.5 * cache.get(48, 2).cost, large_slots=1)
+    reqs = [Request(0, 48, 2, 1, 0.0), Request(1, 48, 2, 2, 0.0)]
+    reqs += [Request(2 + k, 8, 2, 10 + k, 0.0) for k in range(4)]
+    eng.run(reqs)
+    second = max(reqs[:2], key=lambda r: r.admitted)
+    assert all(r.admitted <= second.admitted for r in reqs[2:])
