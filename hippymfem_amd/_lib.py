@@ -25,6 +25,7 @@ HM_ERR_ARG, HM_ERR_HIP, HM_ERR_ROW_LONG, HM_ERR_DUP, HM_ERR_NODIAG, HM_ERR_UNSUP
 #: every symbol ``include/hippymfem_amd.h`` declares
 EXPORTS = ("hm_last_error", "hm_version", "hm_element_matrices", "hm_element_matrices_pair",
            "hm_consistent_block", "hm_element_vectors", "hm_element_hvp", "hm_element_scalars",
+           "hm_point_fields", "hm_element_matrices_pointwise", "hm_element_vectors_pointwise",
            "hm_pattern_build",
            "hm_pattern_place", "hm_pattern_repeats", "hm_pattern_work_bytes",
            "hm_pattern_build_device", "hm_pattern_place_device", "hm_scatter_matrix",
@@ -88,6 +89,10 @@ def lib():
         "hm_element_vectors": (ctypes.c_int, [P, ctypes.c_int, I64, I64, P, P]),
         "hm_element_hvp": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, P, I64, I64, P, P]),
         "hm_element_scalars": (ctypes.c_int, [P, I64, I64, P, P]),
+        "hm_point_fields": (ctypes.c_int, [P, I64, I64, P, P]),
+        "hm_element_matrices_pointwise": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, P, I64,
+                                                         I64, ctypes.c_int, P, P]),
+        "hm_element_vectors_pointwise": (ctypes.c_int, [P, ctypes.c_int, P, I64, I64, P, P]),
         "hm_pattern_build": (ctypes.c_int, [P, P, I64, ctypes.c_int, ctypes.c_int, I64, I64,
                                             ctypes.c_int, ctypes.c_int, P, P, P, P, P, P]),
         "hm_pattern_place": (ctypes.c_int, [P, I64, ctypes.c_int, ctypes.c_int, I64, P, P, P,

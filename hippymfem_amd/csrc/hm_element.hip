@@ -301,6 +301,173 @@ __global__ void __launch_bounds__(256) k_element_scalars(hm_problem P, int64_t e
   if (bad && P.nonfinite) atomicOr(P.nonfinite, 1);
 }
 
+// ---------------------------------------------------------------- caller-supplied densities
+// A density the caller evaluates itself: the fields at every point go out (hm_point_fields),
+// its pointwise derivatives come back in the physical basis (value, d/dx_0..2) and are
+// contracted here, with w |det J| and J^-1 folded in as for the closed-form densities:
+//   D^_vv = w D_vv,  D^_vk = w sum_d D_vd Jinv[k][d],  D^_kv = w sum_d Jinv[k][d] D_dv,
+//   D^_kl = w sum_de Jinv[k][d] D_de Jinv[l][e].
+
+// F (e1-e0, nq, 13): w |det J|, u, grad u, m, grad m, p, grad p (physical gradients).
+__global__ void __launch_bounds__(256) k_point_fields(hm_problem P, int64_t e0, int64_t e1, double* out) {
+  extern __shared__ double lds[];
+  const int nq = P.nq;
+  const WaveElement w = wave_element(P);
+  const int words = 24 + 2 * P.nd_u + P.nd_m;
+  double* Xe = lds + (size_t)((threadIdx.x >> 6) * w.EPW + (w.in_use ? w.slot : 0)) * words;
+  double* ue = Xe + 24;
+  double* pe = ue + P.nd_u;
+  double* me = pe + P.nd_u;
+  const int64_t ne = e1 - e0;
+  const int64_t stride = (int64_t)gridDim.x * (blockDim.x >> 6) * w.EPW;
+  for (int64_t base = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * w.EPW; base < ne;
+       base += stride) {
+    const int64_t el = base + w.slot;
+    const bool act = w.in_use && el < ne;
+    if (act) load_element_lanes(P, e0 + el, w.r, Xe, ue, me, pe);
+    wave_sync();
+    if (act && w.r < nq) {
+      const int q = w.r;
+      PointState S;
+      point_state(P, q, Xe, ue, me, pe, S);
+      const int nu = P.nd_u, nm = P.nd_m;
+      const double* Nu = P.tab_u + (size_t)q * nu;
+      const double* Nm = P.tab_m + (size_t)q * nm;
+      const double* Gm = P.tab_m + (size_t)nq * nm + (size_t)q * nm * 3;
+      double pq = 0, mq = 0, rm[3] = {0, 0, 0};
+      for (int a = 0; a < nu; ++a) pq = fma(Nu[a], pe[a], pq);
+      for (int a = 0; a < nm; ++a) {
+        mq = fma(Nm[a], me[a], mq);
+        for (int k = 0; k < 3; ++k) rm[k] = fma(Gm[a * 3 + k], me[a], rm[k]);
+      }
+      double* F = out + ((size_t)el * nq + q) * 13;
+      F[0] = S.w;
+      F[1] = S.uq;
+      for (int d = 0; d < 3; ++d) F[2 + d] = S.gu[d];
+      F[5] = mq;
+      for (int d = 0; d < 3; ++d) F[6 + d] = S.Ji[0][d] * rm[0] + S.Ji[1][d] * rm[1] + S.Ji[2][d] * rm[2];
+      F[9] = pq;
+      for (int d = 0; d < 3; ++d) F[10 + d] = S.gp[d];
+    }
+    wave_sync();
+  }
+}
+
+// the caller's physical-basis block D (4 x 4) at a point, folded into the reference basis
+__device__ inline void fold_pointwise(const double* D, const PointState& S, double Dh[4][4]) {
+  double L[4][4];  // L[r][c'] = sum_d Jinv-side of the rows: row r of (Jinv-folded D)
+  for (int c = 0; c < 4; ++c) {
+    L[0][c] = D[c];
+    for (int k = 0; k < 3; ++k)
+      L[1 + k][c] = S.Ji[k][0] * D[4 + c] + S.Ji[k][1] * D[8 + c] + S.Ji[k][2] * D[12 + c];
+  }
+  for (int r = 0; r < 4; ++r) {
+    Dh[r][0] = S.w * L[r][0];
+    for (int l = 0; l < 3; ++l)
+      Dh[r][1 + l] = S.w * (L[r][1] * S.Ji[l][0] + L[r][2] * S.Ji[l][1] + L[r][3] * S.Ji[l][2]);
+  }
+}
+
+// Element matrices of block (s, j) from the caller's pointwise blocks Din (e1-e0, nq, 4, 4):
+// the generic contraction of k_element_matrices.
+template <typename OutT>
+__global__ void __launch_bounds__(256) k_element_matrices_pointwise(hm_problem P, int s, int j, const double* Din,
+                                                                    int64_t e0, int64_t e1, OutT* out) {
+  extern __shared__ double lds[];
+  const int nq = P.nq;
+  const int nd_t = s == HM_PARAMETER ? P.nd_m : P.nd_u;
+  const int nd_c = j == HM_PARAMETER ? P.nd_m : P.nd_u;
+  const double* tab_t = s == HM_PARAMETER ? P.tab_m : P.tab_u;
+  const double* tab_c = j == HM_PARAMETER ? P.tab_m : P.tab_u;
+  const bool same = (tab_t == tab_c);
+  double* Pt = lds;
+  double* Pc = same ? Pt : Pt + nq * 4 * nd_t;
+  double* T = Pc + (same ? nq * 4 * nd_t : nq * 4 * nd_c);
+  double* D = T + nq * 4 * nd_c;
+  double* Xe = D + nq * 16;
+  double* ue = Xe + 24;
+  double* pe = ue + P.nd_u;
+  double* me = pe + P.nd_u;
+  load_basis(tab_t, nq, nd_t, Pt);
+  if (!same) load_basis(tab_c, nq, nd_c, Pc);
+  for (int64_t e = e0 + blockIdx.x; e < e1; e += gridDim.x) {
+    __syncthreads();
+    load_element(P, e, Xe, ue, me, pe);
+    __syncthreads();
+    for (int q = threadIdx.x; q < nq; q += blockDim.x) {
+      PointState S;
+      point_state(P, q, Xe, ue, me, pe, S, false);
+      double Dq[4][4];
+      fold_pointwise(Din + ((size_t)(e - e0) * nq + q) * 16, S, Dq);
+      for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) D[q * 16 + r * 4 + c] = Dq[r][c];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nq * 4 * nd_c; i += blockDim.x) {
+      const int b = i % nd_c, r = (i / nd_c) % 4, q = i / (4 * nd_c);
+      double acc = 0;
+      for (int c = 0; c < 4; ++c) acc = fma(D[q * 16 + r * 4 + c], Pc[(q * 4 + c) * nd_c + b], acc);
+      T[i] = acc;
+    }
+    __syncthreads();
+    OutT* Eo = out + (size_t)(e - e0) * nd_t * nd_c;
+    for (int i = threadIdx.x; i < nd_t * nd_c; i += blockDim.x) {
+      const int a = i / nd_c, b = i % nd_c;
+      double acc = 0;
+      for (int q = 0; q < nq; ++q)
+        for (int r = 0; r < 4; ++r) acc = fma(Pt[(q * 4 + r) * nd_t + a], T[(q * 4 + r) * nd_c + b], acc);
+      Eo[i] = (OutT)acc;
+    }
+  }
+}
+
+// Element vectors of slot s from the caller's pointwise first derivatives Vin (e1-e0, nq, 4):
+// sum_q w [V_v N_a + (Jinv V_g) . G_a].
+__global__ void __launch_bounds__(256) k_element_vectors_pointwise(hm_problem P, int s, const double* Vin,
+                                                                   int64_t e0, int64_t e1, double* out) {
+  extern __shared__ double lds[];
+  const int nq = P.nq;
+  const int nd = s == HM_PARAMETER ? P.nd_m : P.nd_u;
+  const double* tab = s == HM_PARAMETER ? P.tab_m : P.tab_u;
+  double* Ps = lds;
+  load_basis(tab, nq, nd, Ps);
+  __syncthreads();
+  const WaveElement w = wave_element(P);
+  const int words = nq * 4 + 24 + 2 * P.nd_u + P.nd_m;
+  double* W = Ps + nq * 4 * nd + (size_t)((threadIdx.x >> 6) * w.EPW + (w.in_use ? w.slot : 0)) * words;
+  double* Xe = W + nq * 4;
+  double* ue = Xe + 24;
+  double* pe = ue + P.nd_u;
+  double* me = pe + P.nd_u;
+  const int64_t ne = e1 - e0;
+  const int64_t stride = (int64_t)gridDim.x * (blockDim.x >> 6) * w.EPW;
+  for (int64_t base = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * w.EPW; base < ne;
+       base += stride) {
+    const int64_t el = base + w.slot;
+    const bool act = w.in_use && el < ne;
+    if (act) load_element_lanes(P, e0 + el, w.r, Xe, ue, me, pe);
+    wave_sync();
+    if (act && w.r < nq) {
+      const int q = w.r;
+      PointState S;
+      point_state(P, q, Xe, ue, me, pe, S, false);
+      const double* V = Vin + ((size_t)el * nq + q) * 4;
+      W[q * 4] = S.w * V[0];
+      for (int k = 0; k < 3; ++k)
+        W[q * 4 + 1 + k] = S.w * (S.Ji[k][0] * V[1] + S.Ji[k][1] * V[2] + S.Ji[k][2] * V[3]);
+    }
+    wave_sync();
+    if (act && w.r < nd) {
+      const int a = w.r;
+      double acc = 0;
+      for (int q = 0; q < nq; ++q)
+        for (int r = 0; r < 4; ++r) acc = fma(Ps[(q * 4 + r) * nd + a], W[q * 4 + r], acc);
+      out[(size_t)el * nd + a] = acc;
+    }
+    wave_sync();
+  }
+}
+
 static int check_problem(const hm_problem* P) {
   HM_REQUIRE(P != nullptr, "no problem given");
   HM_REQUIRE(P->ne >= 0 && P->nq > 0 && P->nq <= 64, "quadrature points per element %d outside 1..64", P->nq);
@@ -513,6 +680,65 @@ extern "C" int hm_element_scalars(const hm_problem* P, int64_t e0, int64_t e1, d
   hipLaunchKernelGGL(k_element_scalars, dim3(grid_for((e1 - e0 + epb - 1) / epb)), dim3(256), bytes,
                      (hipStream_t)stream, *P, e0, e1, out);
   return hip_check("hm_element_scalars");
+}
+
+extern "C" int hm_point_fields(const hm_problem* P, int64_t e0, int64_t e1, double* out, void* stream) {
+  if (int rc = check_problem(P)) return rc;
+  HM_REQUIRE(e0 >= 0 && e0 <= e1 && e1 <= P->ne, "elements outside the batch");
+  HM_REQUIRE(e1 == e0 || out, "no output given");
+  if (e1 == e0) return HM_OK;
+  const int epb = 4 * wave_elements(*P);
+  const size_t bytes = (size_t)epb * (24 + 2 * (size_t)P->nd_u + P->nd_m) * 8;
+  if (int rc = fit_lds((const void*)k_point_fields, bytes)) return rc;
+  hipLaunchKernelGGL(k_point_fields, dim3(grid_for((e1 - e0 + epb - 1) / epb)), dim3(256), bytes,
+                     (hipStream_t)stream, *P, e0, e1, out);
+  return hip_check("hm_point_fields");
+}
+
+extern "C" int hm_element_matrices_pointwise(const hm_problem* P, int i, int j, const double* D, int64_t e0,
+                                             int64_t e1, int precision, void* out, void* stream) {
+  if (int rc = check_problem(P)) return rc;
+  HM_REQUIRE(i >= 0 && i < 3 && j >= 0 && j < 3, "slots (%d, %d) outside 0..2", i, j);
+  HM_REQUIRE(e0 >= 0 && e0 <= e1 && e1 <= P->ne, "elements outside the batch");
+  HM_REQUIRE(precision == HM_FP64 || precision == HM_FP32, "precision %d: FP64 or FP32 results", precision);
+  HM_REQUIRE(e1 == e0 || (D && out), "no blocks or output given");
+  if (e1 == e0) return HM_OK;
+  const int nq = P->nq;
+  const int nd_t = i == HM_PARAMETER ? P->nd_m : P->nd_u;
+  const int nd_c = j == HM_PARAMETER ? P->nd_m : P->nd_u;
+  const bool same = (i == HM_PARAMETER) == (j == HM_PARAMETER);
+  const size_t words = (size_t)nq * 4 * nd_t + (same ? 0 : (size_t)nq * 4 * nd_c) + (size_t)nq * 4 * nd_c +
+                       (size_t)nq * 16 + 24 + 2 * P->nd_u + P->nd_m;
+  const size_t bytes = words * sizeof(double);
+  hipStream_t s = (hipStream_t)stream;
+  if (precision == HM_FP64) {
+    if (int rc = fit_lds((const void*)k_element_matrices_pointwise<double>, bytes)) return rc;
+    hipLaunchKernelGGL(k_element_matrices_pointwise<double>, dim3(grid_for(e1 - e0)), dim3(256), bytes, s, *P,
+                       i, j, D, e0, e1, (double*)out);
+  } else {
+    if (int rc = fit_lds((const void*)k_element_matrices_pointwise<float>, bytes)) return rc;
+    hipLaunchKernelGGL(k_element_matrices_pointwise<float>, dim3(grid_for(e1 - e0)), dim3(256), bytes, s, *P,
+                       i, j, D, e0, e1, (float*)out);
+  }
+  if (int rc = hip_check("hm_element_matrices_pointwise")) return rc;
+  return hm_flag_nonfinite(out, (size_t)(e1 - e0) * nd_t * nd_c, precision == HM_FP64 ? 8 : 4, P->nonfinite, s);
+}
+
+extern "C" int hm_element_vectors_pointwise(const hm_problem* P, int i, const double* V, int64_t e0, int64_t e1,
+                                            double* out, void* stream) {
+  if (int rc = check_problem(P)) return rc;
+  HM_REQUIRE(i >= 0 && i < 3, "slot %d outside 0..2", i);
+  HM_REQUIRE(e0 >= 0 && e0 <= e1 && e1 <= P->ne, "elements outside the batch");
+  HM_REQUIRE(e1 == e0 || (V && out), "no derivatives or output given");
+  if (e1 == e0) return HM_OK;
+  const int nd = i == HM_PARAMETER ? P->nd_m : P->nd_u;
+  const int epb = 4 * wave_elements(*P);
+  const size_t bytes = ((size_t)P->nq * 4 * nd +
+                        (size_t)epb * ((size_t)P->nq * 4 + 24 + 2 * P->nd_u + P->nd_m)) * 8;
+  if (int rc = fit_lds((const void*)k_element_vectors_pointwise, bytes)) return rc;
+  hipLaunchKernelGGL(k_element_vectors_pointwise, dim3(grid_for((e1 - e0 + epb - 1) / epb)), dim3(256), bytes,
+                     (hipStream_t)stream, *P, i, V, e0, e1, out);
+  return hip_check("hm_element_vectors_pointwise");
 }
 
 extern "C" int hm_element_hvp(const hm_problem* P, int i, int j, const double* dir, int64_t e0,

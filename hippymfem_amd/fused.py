@@ -33,6 +33,8 @@ def supported(kernel, i, j, precision):
     """Whether block ``(i, j)`` of ``kernel`` can be assembled by the fused route; ``None``
     when it can, else the reason."""
     mesh = kernel.batches.mesh
+    if kernel.pointwise:
+        return "a caller-supplied density"
     if precision not in PRECISIONS:
         return "precision %s" % precision
     if not hasattr(mesh, "elem_ijk") or not hasattr(mesh, "n"):

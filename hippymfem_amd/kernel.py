@@ -12,10 +12,13 @@ Slots are ``STATE=0`` (u), ``PARAMETER=1`` (m), ``ADJOINT=2`` (p); ``spaces`` is
 ``[Vu, Vm, Vu]``.  Arrays come back as torch tensors on the device.
 """
 
+import contextlib
+
 import numpy as np
 import torch
 
 from . import _lib
+from . import density as _density
 from ._lib import ADJOINT, PARAMETER, STATE
 
 #: Largest number of elements handed to one launch when chunks are asked for
@@ -52,6 +55,9 @@ class QuadratureKernel:
                 spaces[STATE].order != spaces[ADJOINT].order:
             raise ValueError("spaces must be [state, parameter, adjoint], state and adjoint alike")
         self.density = density
+        #: a caller-supplied density (``density.PointwiseDensity``): its pointwise derivatives
+        #: by automatic differentiation, the contraction by the library
+        self.pointwise = bool(getattr(density, "pointwise", False))
         self.spaces = list(spaces)
         self.batches = batches
         self.group = batches.groups[0]
@@ -109,10 +115,18 @@ class QuadratureKernel:
             return [self.consistent_block(i, j, local_arrays, consistent == "symmetric",
                                           precision=precision, elements=elements, out=out,
                                           stream=stream)]
-        P, keep = self.problem(local_arrays, nonfinite)
         a, b = (0, self.group.ne) if elements is None else elements
         code = _lib.PRECISIONS[precision]
         dtype = torch.float64 if code == _lib.FP64 else torch.float32
+        if self.pointwise:
+            if out is None:
+                out = _result_array((b - a, self.nd[i], self.nd[j]), dtype,
+                                    _lib.require_device())
+            with _on(stream):
+                _density.element_matrices(self, i, j, local_arrays, a, b, out, precision,
+                                          nonfinite)
+            return [out]
+        P, keep = self.problem(local_arrays, nonfinite)
         if out is None:
             out = _result_array((b - a, self.nd[i], self.nd[j]), dtype, keep[0].device)
         _lib.check(_lib.lib().hm_element_matrices(
@@ -157,7 +171,7 @@ class QuadratureKernel:
         contractions.  Each result has the bits ``element_matrices`` gives it.  ``None``, with
         nothing written, where the two do not share a pass (another shape, a block that is
         zero for the density, the BF16/FP16 forms)."""
-        if precision not in ("fp64", "fp32"):
+        if precision not in ("fp64", "fp32") or self.pointwise:
             return None
         (i1, j1), (i2, j2) = ij1, ij2
         if (self.nd[i1], self.nd[j1]) != (self.nd[i2], self.nd[j2]):
@@ -224,8 +238,15 @@ class QuadratureKernel:
 
     def element_vectors(self, i, local_arrays, params=(), elements=None, out=None, stream=None):
         """``dR_e / d(slot i)`` ``[(ne, nd_i)]`` (``element_vectors``, ``kernel.py:2070-2078``)."""
-        P, keep = self.problem(local_arrays)
         a, b = (0, self.group.ne) if elements is None else elements
+        if self.pointwise:
+            if out is None:
+                out = torch.empty((b - a, self.nd[i]), dtype=torch.float64,
+                                  device=_lib.require_device())
+            with _on(stream):
+                _density.element_vectors(self, i, local_arrays, a, b, out)
+            return [out]
+        P, keep = self.problem(local_arrays)
         if out is None:
             out = torch.empty((b - a, self.nd[i]), dtype=torch.float64, device=keep[0].device)
         _lib.check(_lib.lib().hm_element_vectors(_lib.ctypes.byref(P), int(i), int(a), int(b),
@@ -239,9 +260,17 @@ class QuadratureKernel:
         ``kernel.py:2129-2136``; the functional of ``_functional``, ``kernel.py:977-1009``):
         ``sum_q w_q |det J| (exp(m) g(u) grad u . grad p - f p)``, summed in point order.
         ``RuntimeError`` where one is not finite."""
+        a, b = (0, self.group.ne) if elements is None else elements
+        if self.pointwise:
+            if out is None:
+                out = torch.empty(b - a, dtype=torch.float64, device=_lib.require_device())
+            with _on(stream):
+                _density.element_values(self, local_arrays, a, b, out)
+            if not bool(torch.isfinite(out).all()):
+                raise RuntimeError("element values: a value is not finite")
+            return [out]
         flag = torch.zeros(1, dtype=torch.int32, device=_lib.require_device())
         P, keep = self.problem(local_arrays, flag)
-        a, b = (0, self.group.ne) if elements is None else elements
         if out is None:
             out = torch.empty(b - a, dtype=torch.float64, device=keep[0].device)
         _lib.check(_lib.lib().hm_element_scalars(_lib.ctypes.byref(P), int(a), int(b),
@@ -256,9 +285,20 @@ class QuadratureKernel:
                     stream=None):
         """Block ``(i, j)`` applied to the slot-``j`` direction element by element
         ``[(ne, nd_i)]`` (``element_hvp``, ``kernel.py:2080-2090``)."""
+        a, b = (0, self.group.ne) if elements is None else elements
+        if self.pointwise:
+            # the element matrices of the caller density times the direction's element dofs
+            E = self.element_matrices(i, j, local_arrays, elements=(a, b), stream=stream)[0]
+            d = _vec(jdir_local, E.device)
+            ed = self.tables[j].device()["edofs"][a:b].long()
+            with _on(stream):
+                y = torch.bmm(E, d[ed].unsqueeze(2)).squeeze(2)
+                if out is not None:
+                    out.copy_(y)
+                    y = out
+            return [y]
         P, keep = self.problem(local_arrays)
         d = _vec(jdir_local, keep[0].device)
-        a, b = (0, self.group.ne) if elements is None else elements
         if out is None:
             out = torch.empty((b - a, self.nd[i]), dtype=torch.float64, device=keep[0].device)
         _lib.check(_lib.lib().hm_element_hvp(_lib.ctypes.byref(P), int(i), int(j), _lib.ptr(d),
@@ -282,6 +322,10 @@ def _result_array(shape, dtype, dev):
             "element batch of %d does not fit on %s: its result is %.2f GB against %.2f GB "
             "free. Assemble it in chunks (element_matrix_chunks, or a callable for "
             "assemble_matrix)." % (shape[0], dev, need / 2 ** 30, free / 2 ** 30)) from exc
+
+
+def _on(stream):
+    return torch.cuda.stream(stream) if stream is not None else contextlib.nullcontext()
 
 
 def _vec(a, dev):

@@ -131,6 +131,23 @@ int hm_element_vectors(const hm_problem* P, int i, int64_t e0, int64_t e1, doubl
  * GroupKernel._functional (kernel.py:977-1009). */
 int hm_element_scalars(const hm_problem* P, int64_t e0, int64_t e1, double* out, void* stream);
 
+/* Caller-supplied densities (QuadratureKernel(density, ...), hippymfem/fem/kernel.py:1917-1937,
+ * whose pointwise derivatives the reference forms by automatic differentiation in
+ * _point_blocks, kernel.py:1581-1646): the caller differentiates its density at the points
+ * and the library contracts the result.
+ * hm_point_fields: F (e1-e0, nq, 13) fp64 = w |det J|, u, grad u, m, grad m, p, grad p at
+ *   every point (physical gradients).
+ * hm_element_matrices_pointwise: block (i, j) element matrices from D (e1-e0, nq, 4, 4) fp64,
+ *   d2r / d(slot i value, grad) d(slot j value, grad) in the physical basis; out as
+ *   hm_element_matrices (FP64 or FP32 results); sets *nonfinite.
+ * hm_element_vectors_pointwise: slot i element vectors from V (e1-e0, nq, 4) fp64,
+ *   dr / d(slot i value, grad); out (e1-e0, nd_i) fp64. */
+int hm_point_fields(const hm_problem* P, int64_t e0, int64_t e1, double* out, void* stream);
+int hm_element_matrices_pointwise(const hm_problem* P, int i, int j, const double* D, int64_t e0,
+                                  int64_t e1, int precision, void* out, void* stream);
+int hm_element_vectors_pointwise(const hm_problem* P, int i, const double* V, int64_t e0,
+                                 int64_t e1, double* out, void* stream);
+
 /* Block (i, j) applied element by element to the slot-j dof vector dir:
  * out (e1-e0, nd_i) fp64.  Replaces GroupKernel.hvp_block (hippymfem/fem/kernel.py:1756-1782). */
 int hm_element_hvp(const hm_problem* P, int i, int j, const double* dir, int64_t e0,
