@@ -86,23 +86,27 @@ class ElementGroup:
     vertex coordinates, element vertices, trilinear-basis gradients and weights are made on
     first use and kept."""
 
-    def __init__(self, mesh, quadrature_degree):
+    def __init__(self, mesh, quadrature_degree, elems=None):
         self.mesh = mesh
         self.degree = int(quadrature_degree)
         self.points, self.w = integration_rule(self.degree)
         self.nq = self.points.shape[0]
         self.nq1 = int(self.degree) // 2 + 1
         self.points1, _ = gauss_legendre(self.nq1)
-        self.ne = mesh.ne
-        self.elems = np.arange(mesh.ne)
+        #: the mesh elements of the group (ascending), all of them by default
+        self.elems = np.arange(mesh.ne) if elems is None else np.asarray(elems, dtype=np.int64)
+        self.whole = elems is None or (self.elems.size == mesh.ne and
+                                       np.array_equal(self.elems, np.arange(mesh.ne)))
+        self.ne = int(self.elems.size)
         _, self.Gx = vertex_basis(self.points)
         self._dev = None
 
     def device(self):
         if self._dev is None:
+            ev = self.mesh.elem_verts if self.whole else self.mesh.elem_verts[self.elems]
             self._dev = dict(
                 verts=_dev(self.mesh.verts, torch.float64),
-                elem_verts=_dev(self.mesh.elem_verts, torch.int32),
+                elem_verts=_dev(ev, torch.int32),
                 tab_x=_dev(self.Gx, torch.float64),
                 wq=_dev(self.w, torch.float64))
         return self._dev
@@ -122,7 +126,8 @@ class SpaceTables:
         self.nd_total = space.nd
         self.vdim = 1
         self.N, self.G = basis_tables(space.order, group.points, space.local_nodes)
-        self.edofs = space.edofs
+        self.edofs = space.edofs if getattr(group, "whole", True) else \
+            np.ascontiguousarray(space.edofs[group.elems])
         # tensor structure: the 1-D basis at the 1-D points and the local dof of every
         # tensor index a1 + n1 (a2 + n1 a3)
         self.n1 = space.order + 1
@@ -146,13 +151,25 @@ class SpaceTables:
 
 
 class MeshBatches:
-    """The element groups of a mesh (``MeshBatches``, ``elementbatch.py:600-698``): one
-    group on a hexahedral box mesh."""
+    """The element groups of a mesh (``MeshBatches``, ``elementbatch.py:600-698``): one group
+    of every hexahedron by default, or ``groups`` contiguous ranges of elements (the
+    reference's groups by geometry or attribute; here all hexahedra, one quadrature rule).
+    A kernel takes one group: ``subset(g)`` is the batches of group ``g`` alone."""
 
-    def __init__(self, mesh, quadrature_degree):
+    def __init__(self, mesh, quadrature_degree, groups=1):
         self.mesh = mesh
-        self.groups = [ElementGroup(mesh, quadrature_degree)]
+        if int(groups) == 1:
+            self.groups = [ElementGroup(mesh, quadrature_degree)]
+        else:
+            self.groups = [ElementGroup(mesh, quadrature_degree, elems=r)
+                           for r in np.array_split(np.arange(mesh.ne), int(groups))]
         self._tables = {}
+
+    def subset(self, g):
+        """The batches of group ``g`` alone (same mesh, group and tables)."""
+        sub = MeshBatches.__new__(MeshBatches)
+        sub.mesh, sub.groups, sub._tables = self.mesh, [self.groups[g]], {}
+        return sub
 
     def tables(self, space):
         key = id(space)

@@ -39,6 +39,8 @@ def supported(kernel, i, j, precision):
         return "precision %s" % precision
     if not hasattr(mesh, "elem_ijk") or not hasattr(mesh, "n"):
         return "not a box mesh"
+    if not getattr(kernel.group, "whole", True):
+        return "an element group that is not the whole box"
     if kernel.group.nq1 ** 3 != kernel.group.nq or kernel.group.nq1 not in (3, 4):
         return "no tensor rule of 3 or 4 points a direction"
     for s in (kernel.spaces[i], kernel.spaces[j]):

@@ -60,6 +60,9 @@ class QuadratureKernel:
         self.pointwise = bool(getattr(density, "pointwise", False))
         self.spaces = list(spaces)
         self.batches = batches
+        if len(batches.groups) != 1:
+            raise NotImplementedError("a kernel takes one element group; use one kernel a group "
+                                      "(MeshBatches.subset) and assemble their arrays together")
         self.group = batches.groups[0]
         self.tables = [batches.tables(s)[0] for s in spaces]
         self.nd = [t.nd for t in self.tables]

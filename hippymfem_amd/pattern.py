@@ -105,22 +105,30 @@ class ScatterPattern:
 
     def __init__(self, test_tables, trial_tables, test_space=None, trial_space=None,
                  nthreads=0, build="auto"):
-        if len(test_tables) != len(trial_tables):
+        if len(test_tables) != len(trial_tables) or not test_tables:
             raise ValueError("test and trial tables must cover the same groups")
-        if len(test_tables) != 1:
-            raise NotImplementedError("one element group (one element geometry) is supported")
         tt, tr = test_tables[0], trial_tables[0]
         test_space = test_space if test_space is not None else tt.space
         trial_space = trial_space if trial_space is not None else tr.space
         self.nrow, self.ncol = int(test_space.ndof), int(trial_space.ndof)
-        self.shapes = [(int(tt.group.ne), int(tt.nd_total), int(tr.nd_total))]
+        # groups: their entries in group order (entry k of group g at its offset), the
+        # elements of all groups as one run of elements
+        self.shapes = [(int(a.group.ne), int(a.nd_total), int(b.nd_total))
+                       for a, b in zip(test_tables, trial_tables)]
+        if len({(a, b) for _, a, b in self.shapes}) != 1:
+            raise NotImplementedError("element groups of different dof counts (element "
+                                      "geometries) in one pattern")
         self.sizes = [ne * a * b for ne, a, b in self.shapes]
         self.nent = int(sum(self.sizes))
-        self.ne, self.nd_t, self.nd_c = self.shapes[0]
+        self.ne = int(sum(ne for ne, _, _ in self.shapes))
+        _, self.nd_t, self.nd_c = self.shapes[0]
+        self.group_start = np.concatenate([[0], np.cumsum([ne for ne, _, _ in self.shapes])])
         self.sign = None
         self.perm = None
-        self.tdofs = np.ascontiguousarray(tt.edofs, dtype=np.int32)
-        self.cdofs = np.ascontiguousarray(tr.edofs, dtype=np.int32)
+        self.tdofs = np.ascontiguousarray(np.concatenate([np.asarray(t.edofs).reshape(
+            -1, self.nd_t) for t in test_tables]), dtype=np.int32)
+        self.cdofs = np.ascontiguousarray(np.concatenate([np.asarray(t.edofs).reshape(
+            -1, self.nd_c) for t in trial_tables]), dtype=np.int32)
         self.diag_first = self.nrow == self.ncol
         if max(self.nrow, self.ncol) > (1 << 31) - 1:
             raise ValueError("%d rows and %d columns do not fit 32-bit local indices"
@@ -341,7 +349,10 @@ class ScatterPattern:
         if len(element_matrices) != len(self.shapes):
             raise ValueError("expected %d element arrays, got %d"
                              % (len(self.shapes), len(element_matrices)))
-        E = _as_device(element_matrices[0])
+        # the groups' arrays as one run of elements (one scatter: every sum in entry order,
+        # rounded once)
+        E = _as_device(element_matrices[0]) if len(element_matrices) == 1 else torch.cat(
+            [_as_device(a).reshape(-1) for a in element_matrices])
         if E.numel() != self.nent:
             raise ValueError("element arrays hold %d entries, pattern expects %d"
                              % (E.numel(), self.nent))
@@ -375,6 +386,8 @@ class ScatterPattern:
     def fused_add(self, acc, maps, g, a, bnd, arr, plan=None, stream=None):
         """Add the element matrices of elements ``[a, bnd)`` of group ``g``, in order
         (``fused_add``, ``pattern.py:864-886``)."""
+        off = int(self.group_start[g])
+        a, bnd = a + off, bnd + off
         E = _as_device(arr)
         if E.numel() != (bnd - a) * self.nd_t * self.nd_c:
             raise ValueError("chunk [%d, %d) of group %d holds %d entries, pattern expects %d"
@@ -406,12 +419,15 @@ class VectorPattern:
     ``pattern.py:1156-1233``)."""
 
     def __init__(self, tables, space=None):
-        if len(tables) != 1:
-            raise NotImplementedError("one element group (one element geometry) is supported")
         t = tables[0]
         self.n = int((space if space is not None else t.space).ndof)
-        self.ne, self.nd = int(t.group.ne), int(t.nd_total)
-        self.edofs = np.ascontiguousarray(t.edofs, dtype=np.int32)
+        if len({int(x.nd_total) for x in tables}) != 1:
+            raise NotImplementedError("element groups of different dof counts in one pattern")
+        self.nd = int(t.nd_total)
+        self.ne = int(sum(int(x.group.ne) for x in tables))
+        self.ngroups = len(tables)
+        self.edofs = np.ascontiguousarray(np.concatenate(
+            [np.asarray(x.edofs).reshape(-1, self.nd) for x in tables]), dtype=np.int32)
         zero = np.zeros((self.ne, 1), np.int32)
         _, rptr, runs, _, _, _ = build_tables(self.edofs, zero, self.n, 1, False)
         self.rptr, self.runs = rptr, runs
@@ -429,7 +445,8 @@ class VectorPattern:
         """The local dof array of summed element vectors, in ascending entry order
         (``VectorPattern.scatter``, ``pattern.py:1186-1209``); ``zero_rows`` (int8, one a
         dof) clears the essential entries (``zero_essential``, ``bcs.py:80-102``)."""
-        V = _as_device(element_vectors[0])
+        V = _as_device(element_vectors[0]) if len(element_vectors) == 1 else torch.cat(
+            [_as_device(v).reshape(-1) for v in element_vectors])
         a, bnd = (0, self.ne) if elements is None else elements
         if V.numel() != (bnd - a) * self.nd:
             raise ValueError("element vectors hold %d entries, pattern expects %d"

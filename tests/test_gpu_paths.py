@@ -263,3 +263,43 @@ def test_element_batch_over_budget_streams_in_chunks(gpu, nonlinear, monkeypatch
     m[W.Vm.ndof // 2:] = 800.0
     with pytest.raises(RuntimeError):
         prob.block(ADJOINT, STATE, [W.u, m, W.p])
+
+
+def test_element_groups_assemble_as_one(gpu):
+    """Two element groups (contiguous element ranges, one kernel each): their element
+    matrices are the whole batch's rows, and the block and the residual assembled from the
+    two groups' arrays (entries in group order, ``pattern.py:437-546``) are the one-group
+    ones bit for bit; a chunked pass over the groups gives the same bits."""
+    from hippymfem_amd.assemble import assemble_vector
+    from hippymfem_amd.elementbatch import MeshBatches
+    from hippymfem_amd.kernel import QuadratureKernel
+
+    W = Workload(4, perturb=0.2, nonlinear=True)
+    B2 = MeshBatches(W.mesh, 4, groups=2)
+    assert [g.ne for g in B2.groups] == [32, 32]
+    with pytest.raises(NotImplementedError):
+        QuadratureKernel(W.density, W.spaces, B2)
+    kernels = [QuadratureKernel(W.density, W.spaces, B2.subset(g)) for g in range(2)]
+    loc = W.locals()
+    whole = W.kernel.element_matrices(ADJOINT, STATE, loc)[0]
+    E = [k.element_matrices(ADJOINT, STATE, loc)[0] for k in kernels]
+    assert torch.equal(E[0], whole[:32]) and torch.equal(E[1], whole[32:])
+    A1 = assemble_matrix(W.Vu, W.Vu, W.groups, [whole], W.mesh.ne, test_ess=W.ess)
+    A2 = assemble_matrix(W.Vu, W.Vu, B2.groups, E, W.mesh.ne, test_ess=W.ess)
+    assert np.array_equal(np.asarray(A1.pattern.indptr), np.asarray(A2.pattern.indptr))
+    assert np.array_equal(np.asarray(A1.pattern.indices), np.asarray(A2.pattern.indices))
+    assert torch.equal(A1.data, A2.data)
+
+    def chunks():
+        for g, k in enumerate(kernels):
+            for a in range(0, 32, 12):
+                b = min(a + 12, 32)
+                yield g, a, b, k.element_matrices(ADJOINT, STATE, loc, elements=(a, b))[0]
+
+    A3 = assemble_matrix(W.Vu, W.Vu, B2.groups, chunks, W.mesh.ne, test_ess=W.ess)
+    assert torch.equal(A1.data, A3.data)
+    v1 = assemble_vector(W.Vu, W.groups, W.kernel.element_vectors(ADJOINT, loc), W.mesh.ne,
+                         ess=W.ess)
+    v2 = assemble_vector(W.Vu, B2.groups, [k.element_vectors(ADJOINT, loc)[0] for k in kernels],
+                         W.mesh.ne, ess=W.ess)
+    assert torch.equal(v1, v2)
