@@ -173,6 +173,17 @@ class BlockAssembly:
                                       dtype=self.e_dtype, device=self.data.device)
         return self._E
 
+    def _e_fits(self):
+        """Whether the whole batch's element matrices fit in half of the free device memory
+        (and the element budget)."""
+        k = self.kernel
+        size = torch.empty((), dtype=self.e_dtype).element_size()
+        want = k.group.ne * k.nd[self.i] * k.nd[self.j] * size
+        budget = _budget()
+        if budget is not None and want > budget:
+            return False
+        return want <= 0.5 * torch.cuda.mem_get_info(self.data.device)[0]
+
     @property
     def dplan(self):
         """The scatter's device plan (codes in run order, row modes), made at the first
@@ -202,10 +213,19 @@ class BlockAssembly:
                 why = "values of another precision than the element matrices"
             elif self.plan.diag_policy == "keep" and self.plan.diag is not None:
                 why = "diag_policy 'keep'"
+            elif self.route == "auto" and self.precision != "fp64" and self._e_fits():
+                # single precision: FP32 element matrices halve the two-pass route's traffic,
+                # while the fused kernel's time is its FP64 row sums and point work (64^3 P2 A:
+                # 0.93 ms two-pass against 1.08 ms fused in FP32)
+                why = "single precision (the two-pass route is faster)"
+            elif self.route == "auto" and self.kernel.group.nq1 > 3:
+                # the 4-point rule: 64 points' pairs in LDS leave the fused kernel small tiles
+                # (64^3 P2 A, degree 6: 2.25 ms fused against 1.71 ms two-pass)
+                why = "a 4-point rule (the two-pass route is faster)"
             elif self.route == "auto" and self.kernel.spaces[self.j].order < 2:
                 # a Q1 trial space: the element matrices are small against the point work
                 # the fused kernel repeats for its halo, and the two-pass route takes two such
-                # blocks from one element pass (64^3 P2: C 1.04 ms fused against 0.50 ms a
+                # blocks from one element pass (64^3 P2: C 1.04 ms fused against 0.46 ms a
                 # block for C and W_um together, W_mm 0.78 against 0.36 ms)
                 why = "a Q1 trial space (the two-pass route is faster)"
             else:
