@@ -56,9 +56,12 @@ class HmScatterPlan(ctypes.Structure):
 class HmFusedPlan(ctypes.Structure):
     """``hm_fused_plan`` of ``include/hippymfem_amd.h``."""
     _fields_ = [("nx", I32), ("ny", I32), ("nz", I32), ("bx", I32), ("by", I32),
-                ("reserved", I32), ("hdr", P), ("hdr_off", P), ("sigma", P), ("sig_off", P),
+                ("flags", I32), ("hdr", P), ("hdr_off", P), ("sigma", P), ("sig_off", P),
                 ("diag", DBL)]
 
+
+#: ``hm_fused_plan.flags``: every element an axis-aligned box (``HM_FUSED_ALIGNED``)
+FUSED_ALIGNED = 1
 
 #: int32 words of a row header (``HM_ROWHEAD``)
 ROWHEAD = 12

@@ -65,6 +65,11 @@ struct FusedGeom {
   int ws[SF_MAXPAIR];
   int nslot;
   uint32_t wfirst;
+  // the pairs of every third-direction type tt (test factor d/dxi_3 or not, trial likewise) in
+  // ascending order, for an element of the general form ([0]) and an axis-aligned one ([1]):
+  // d1 | d2 << 2 | slot << 4, with d1, d2 the derivative flags of the first two directions
+  int pl[2][4][SF_MAXPAIR];
+  int pn[2][4];
 };
 
 // Sizes of a tile's LDS, for the launch and the host's choice of tile.
@@ -79,8 +84,9 @@ __host__ __device__ inline int fu_width(int l, int pt, int n, int pc) {
   return pc * (shared ? 2 : 1) + 1;
 }
 
-template <int NT, int NC, int NQ>
-__host__ inline FusedSizes fu_sizes(int bx, int by, int nslot, int nd_u, int nd_m, bool fields, int nz_max) {
+template <typename CT, int NT, int NC, int NQ>
+__host__ inline FusedSizes fu_sizes(int bx, int by, int nslot, int nd_u, int nd_m, bool fields, bool aligned,
+                                    int nz_max) {
   constexpr int pt = NT - 1, pc = NC - 1, NQ3 = NQ * NQ * NQ;
   FusedSizes z{};
   z.m_max = (bx + 1) * (by + 1);
@@ -89,8 +95,11 @@ __host__ inline FusedSizes fu_sizes(int bx, int by, int nslot, int nd_u, int nd_
   z.sx_max = bx * (2 * pc + 1) + (pt - 1) * bx * (pc + 1) + (pc + 1);
   z.sy_max = by * (2 * pc + 1) + (pt - 1) * by * (pc + 1) + (pc + 1);
   z.dw = (24 + nd_m + (fields ? 2 * nd_u : 0) + 1) & ~1;
-  z.ew = ((nslot * NQ3 + 1) & ~1) + z.dw;
-  z.table_words = (size_t)sf_table_words(nd_u, nd_m, fields, NQ) + 4 * NT * NC * NQ;
+  // an element's pairs in the contraction's type (words of FP64, even), then its data
+  z.ew = (int)((((size_t)nslot * NQ3 * sizeof(CT) + 15) & ~(size_t)15) / sizeof(double)) + z.dw;
+  // the tables; the trilinear map's gradients only where an element may not be axis-aligned
+  z.table_words = (size_t)sf_table_words(nd_u, nd_m, fields, NQ) - (aligned ? 24 * NQ * NQ * NQ : 0) +
+                  4 * NT * NC * NQ;
   z.elem_words = (size_t)z.m_max * z.ew;
   const size_t s2d = (size_t)z.sx_max * z.sy_max;
   // even word counts, so that the staged maps after the planes start 16-byte aligned
@@ -102,13 +111,12 @@ __host__ inline FusedSizes fu_sizes(int bx, int by, int nslot, int nd_u, int nd_
   return z;
 }
 
-template <typename CT, int NT, int NC, int QP, int NQ, typename OT, int BS>
+template <typename CT, int NT, int NC, int QP, int NQ, typename OT, int BS, bool AL>
 __global__ void __launch_bounds__(BS) k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G,
                                               FusedSizes Z, const double* __restrict__ t1t,
                                               const double* __restrict__ t1c, OT* __restrict__ data) {
   constexpr int pt = NT - 1, pc = NC - 1;
   constexpr int NQ2 = NQ * NQ, NQ3 = NQ2 * NQ;
-  constexpr int NWAVE = BS / 64;
   static_assert(8 * FU_MAXM <= BS, "eight lanes an element slot");
   extern __shared__ double lds[];
   __shared__ int32_t UT[32];
@@ -125,12 +133,11 @@ __global__ void __launch_bounds__(BS) k_fused(const hm_problem P, int s, int j, 
   // derived from it are recomputed there rather than kept live across the contraction, whose
   // registers they would otherwise spill to scratch memory (a global-latency reload inside
   // every barrier-bound step)
-  int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  int tid = threadIdx.x, lane = tid & 63;
   auto rederive = [&]() {
     int t = (int)threadIdx.x;
     asm volatile("" : "+v"(t));
     tid = t;
-    wave = t >> 6;
     lane = t & 63;
   };
   // ------------------------------------------------------------------ the tile
@@ -149,19 +156,21 @@ __global__ void __launch_bounds__(BS) k_fused(const hm_problem P, int s, int j, 
   const int64_t plane_rows = (int64_t)(pt * G.nx + 1) * (pt * G.ny + 1);
 
   // ------------------------------------------------------------------ LDS
-  double* TX = lds;
-  double* TM = TX + NQ3 * 24;
+  // the tables: the parameter basis, the weights, the state's 1-D basis (fields), the 1-D table
+  // products, and (general form) the trilinear map's gradients
+  double* TM = lds;
   double* TW = TM + NQ3 * nm;
   double* T1U = TW + NQ3;
-  double* PT = lds + sf_table_words(nu, nm, fields, NQ);
+  double* TX = lds + sf_table_words(nu, nm, fields, NQ) - NQ3 * 24;
+  double* PT = AL ? TX : TX + NQ3 * 24;
   double* EL = lds + Z.table_words;                  // element regions
   double* BUF0 = EL + Z.elem_words;                  // plane buffers
   double* BUF1 = BUF0 + Z.plane0_words;
   uint8_t* SIG = (uint8_t*)(BUF1 + Z.plane1_words);  // staged stencil maps, FU_SLOTS planes
   int2* HDR = (int2*)(SIG + FU_SLOTS * Z.sig_bytes); // staged row headers, FU_SLOTS planes
   int64_t* SOFF = (int64_t*)(HDR + FU_SLOTS * Z.rows_max);  // the chunk's planes' map starts
-  const int wpair = (G.nslot * NQ3 + 1) & ~1;
-  auto Wreg = [&](int m) { return EL + (size_t)m * Z.ew; };
+  const int wpair = (int)((((size_t)G.nslot * NQ3 * sizeof(CT) + 15) & ~(size_t)15) / sizeof(double));
+  auto Wreg = [&](int m) { return (CT*)(EL + (size_t)m * Z.ew); };
   auto Dreg = [&](int m) { return EL + (size_t)m * Z.ew + wpair; };
 
   {
@@ -169,7 +178,8 @@ __global__ void __launch_bounds__(BS) k_fused(const hm_problem P, int s, int j, 
       const int q = i % NQ, b = (i / NQ) % NC, a = (i / (NQ * NC)) % NT, dd = i / (NQ * NC * NT);
       PT[i] = t1t[((dd >> 1) * NT + a) * NQ + q] * t1c[((dd & 1) * NC + b) * NQ + q];
     }
-    for (int i = tid; i < NQ3 * 24; i += BS) TX[(i % 24) * NQ3 + i / 24] = P.tab_x[i];
+    if constexpr (!AL)
+      for (int i = tid; i < NQ3 * 24; i += BS) TX[(i % 24) * NQ3 + i / 24] = P.tab_x[i];
     for (int i = tid; i < NQ3 * nm; i += BS) TM[(i % nm) * NQ3 + i / nm] = P.tab_m[i];
     for (int i = tid; i < NQ3; i += BS) TW[i] = P.wq[i];
     if (fields) {
@@ -400,7 +410,6 @@ __global__ void __launch_bounds__(BS) k_fused(const hm_problem P, int s, int j, 
   };
 
   // ------------------------------------------------------------------ the sweep
-  const uint32_t every = SP.npair >= 32 ? ~0u : (1u << SP.npair) - 1u;
   bool bad = false;
   CT acc[NC][NT][NC];
   load_ids(zs);
@@ -416,8 +425,8 @@ __global__ void __launch_bounds__(BS) k_fused(const hm_problem P, int s, int j, 
     // each element's alignment (every vertex coordinate that of the vertex with the same index
     // along its axis: J diagonal, the pairs of two different derivatives left out of its
     // contraction, as the element kernel does; lane v of the eight lanes of element m checks
-    // vertex v, a ballot joins them)
-    {
+    // vertex v, a ballot joins them).  On a mesh of axis-aligned elements (AL) every element is.
+    if constexpr (!AL) {
       const int m = tid >> 3, v = tid & 7;
       bool ok = true;
       if (m < M) {
@@ -427,24 +436,24 @@ __global__ void __launch_bounds__(BS) k_fused(const hm_problem P, int s, int j, 
       }
       const uint64_t all = __ballot(ok);
       if (m < M && v == 0) EFLAG[m] = ((all >> (lane & 56)) & 0xFFu) == 0xFFu ? 1 : 0;
+      __syncthreads();
     }
-    __syncthreads();
     // 2. (element, point) pairs: geometry, fields, pointwise pairs (one LDS slot for the pairs
     // of equal values)
     for (int t = tid; t < M * NQ3; t += BS) {
       const int m = t / NQ3, q = t - m * NQ3;
       const double* D = Dreg(m);
-      double* We = Wreg(m);
+      CT* We = Wreg(m);
       PointState S;
       sf_point_state<NQ>(P, q, D, D + 24 + nm, D + 24, D + 24 + nm + nu, fields, TX, TM, TW, T1U, S,
-                         EFLAG[m] != 0);
+                         AL || EFLAG[m] != 0);
       double Db[4][4];
       point_block(s, j, S, Db);
       int p = 0;
 #pragma unroll
       for (int k = 0; k < 16; ++k)
         if ((SP.kmask >> k) & 1) {
-          if (G.wfirst & (1u << p)) We[G.ws[p] * NQ3 + q] = Db[k >> 2][k & 3];
+          if (G.wfirst & (1u << p)) We[G.ws[p] * NQ3 + q] = (CT)Db[k >> 2][k & 3];
           ++p;
         }
     }
@@ -468,22 +477,23 @@ __global__ void __launch_bounds__(BS) k_fused(const hm_problem P, int s, int j, 
 #pragma unroll
         for (int b3 = 0; b3 < NC; ++b3) acc[i][a3][b3] = 0;
     if (has) {
-      const uint32_t live = every & (EFLAG[m] ? ~(uint32_t)SP.offdiag : ~0u);
-      const double* Wl = Wreg(m);
+      // the element's pairs of each third-direction type, from the plan's lists (the aligned
+      // element's leave out the pairs of two different reference derivatives)
+      const int kind = (AL || EFLAG[m]) ? 1 : 0;
+      const CT* Wl = Wreg(m);
 #pragma unroll 1
       for (int t = 0; t < 4; ++t) {
-        const uint32_t tm = live & (uint32_t)SP.tmask[t];
-        if (tm == 0) continue;
+        const int np = G.pn[kind][t];
+        if (np == 0) continue;
         CT s2[NC][NQ];
 #pragma unroll
         for (int i = 0; i < NC; ++i)
 #pragma unroll
           for (int q = 0; q < NQ; ++q) s2[i][q] = 0;
-        for (int p = 0; p < SP.npair; ++p) {
-          if (!((tm >> p) & 1)) continue;
-          const int d1 = ((SP.pr[p] == 1) << 1) | (SP.pc[p] == 1);
-          const int d2 = ((SP.pr[p] == 2) << 1) | (SP.pc[p] == 2);
-          const double* wv = Wl + G.ws[p] * NQ3;
+        for (int k = 0; k < np; ++k) {
+          const int pw = G.pl[kind][t][k];
+          const int d1 = pw & 3, d2 = (pw >> 2) & 3;
+          const CT* wv = Wl + (pw >> 4) * NQ3;
           const double* f1 = PT + ((d1 * NT + a1) * NC + b1) * NQ;
           double gd[NQ];
 #pragma unroll
@@ -504,13 +514,13 @@ __global__ void __launch_bounds__(BS) k_fused(const hm_problem P, int s, int j, 
             CT s1[NQ];
 #pragma unroll
             for (int q2 = 0; q2 < NQ; ++q2) {
-              const double* wr = wv + NQ * q2 + NQ2 * q3;
+              const CT* wr = wv + NQ * q2 + NQ2 * q3;
               if constexpr (NQ == 3) {
-                s1[q2] = fma(g[2], (CT)wr[2], fma(g[1], (CT)wr[1], g[0] * (CT)wr[0]));
+                s1[q2] = fma(g[2], wr[2], fma(g[1], wr[1], g[0] * wr[0]));
               } else {
-                CT a = g[0] * (CT)wr[0];
+                CT a = g[0] * wr[0];
 #pragma unroll
-                for (int q1 = 1; q1 < NQ; ++q1) a = fma(g[q1], (CT)wr[q1], a);
+                for (int q1 = 1; q1 < NQ; ++q1) a = fma(g[q1], wr[q1], a);
                 s1[q2] = a;
               }
             }
@@ -647,13 +657,13 @@ __global__ void __launch_bounds__(BS) k_fused(const hm_problem P, int s, int j, 
   if (bad && P.nonfinite) atomicOr(P.nonfinite, 1);
 }
 
-template <int NT, int NC, int NQ>
-static FusedSizes fu_pick(const hm_problem& P, const SumfactPlan& SP, int nslot, int bs, size_t limit, int* bx,
-                          int* by) {
+template <typename CT, int NT, int NC, int NQ>
+static FusedSizes fu_pick(const hm_problem& P, const SumfactPlan& SP, int nslot, bool aligned, int bs, size_t limit,
+                          int* bx, int* by) {
   static const int cand[][2] = {{4, 4}, {4, 3}, {3, 3}, {3, 2}, {2, 2}, {2, 1}, {1, 1}};
   FusedSizes z{};
   for (const auto& c : cand) {
-    z = fu_sizes<NT, NC, NQ>(c[0], c[1], nslot, P.nd_u, P.nd_m, SP.need_fields != 0, FU_MAXZ);
+    z = fu_sizes<CT, NT, NC, NQ>(c[0], c[1], nslot, P.nd_u, P.nd_m, SP.need_fields != 0, aligned, FU_MAXZ);
     const int items = (z.m_max * (8 + P.nd_m + (SP.need_fields ? P.nd_u : 0)) + bs - 1) / bs;
     const int sigv = (int)((FU_SLOTS * z.sig_bytes / 16 + bs - 1) / bs);
     if (z.bytes <= limit && z.t_max <= bs && z.t_max <= FU_MAXT && z.m_max <= FU_MAXM && items <= FU_ITEMS &&
@@ -683,21 +693,27 @@ static size_t lds_limit() {
 
 // The LDS slots of the pointwise pairs: a pair whose values are those of an earlier one takes
 // its slot (the reference-gradient pairs (k, l) and (l, k), whose weights w g J^-1 J^-T are
-// symmetric to the bit, and every (r, c), (c, r) of a block of a slot with itself).
+// symmetric to the bit, and every (r, c), (c, r) of a block of a slot with itself).  On a mesh of
+// axis-aligned elements the pairs of two different reference derivatives are left out of every
+// element's contraction and take no slot (-1).
 struct FuSlots {
   int ws[SF_MAXPAIR];
   int nslot;
   uint32_t wfirst;
 };
 
-static FuSlots fu_slots(const SumfactPlan& SP, int i, int j) {
+static FuSlots fu_slots(const SumfactPlan& SP, int i, int j, bool aligned) {
   FuSlots W{};
   for (int p = 0; p < SP.npair; ++p) {
     const int r = SP.pr[p], c = SP.pc[p];
+    if (aligned && ((SP.offdiag >> p) & 1)) {
+      W.ws[p] = -1;
+      continue;
+    }
     int q = -1;
     if (r > c && ((r >= 1 && c >= 1) || i == j))
       for (int t = 0; t < p; ++t)
-        if (SP.pr[t] == c && SP.pc[t] == r) q = t;
+        if (SP.pr[t] == c && SP.pc[t] == r && W.ws[t] >= 0) q = t;
     if (q >= 0) {
       W.ws[p] = W.ws[q];
     } else {
@@ -726,24 +742,28 @@ static int fu_plan(const hm_problem* P, int i, int j, SumfactPlan& SP, int* nt, 
   return HM_OK;
 }
 
-template <int NT, int NC, int NQ>
-static int fu_tile(const hm_problem* P, const SumfactPlan& SP, int nslot, int* bx, int* by) {
+// The tile of block (i, j): the largest candidate that fits the general form (every element
+// tested for its alignment), so that a plan made for it serves a mesh of either kind.
+template <typename CT, int NT, int NC, int NQ>
+static int fu_tile(const hm_problem* P, const SumfactPlan& SP, int i, int j, int* bx, int* by) {
   constexpr int BS = NQ == 3 ? 768 : 512;
-  fu_pick<NT, NC, NQ>(*P, SP, nslot, BS, lds_limit(), bx, by);
+  fu_pick<CT, NT, NC, NQ>(*P, SP, fu_slots(SP, i, j, false).nslot, false, BS, lds_limit(), bx, by);
   if (*bx == 0) HM_FAIL(HM_ERR_UNSUPPORTED, "no tile of the fused assembly fits the LDS");
   return HM_OK;
 }
 
-template <typename CT, int NT, int NC, int QP, int NQ, typename OT>
+template <typename CT, int NT, int NC, int QP, int NQ, typename OT, bool AL>
 static int fu_launch(const hm_problem& P, int i, int j, const SumfactPlan& SP, const hm_fused_plan& F, void* data,
                      hipStream_t st) {
   constexpr int BS = NQ == 3 ? 768 : 512;
-  const FuSlots W = fu_slots(SP, i, j);
+  const FuSlots W = fu_slots(SP, i, j, AL);
   int bx = 0, by = 0;
-  const FusedSizes Z = fu_pick<NT, NC, NQ>(P, SP, W.nslot, BS, lds_limit(), &bx, &by);
+  fu_pick<CT, NT, NC, NQ>(P, SP, fu_slots(SP, i, j, false).nslot, false, BS, lds_limit(), &bx, &by);
   if (bx != F.bx || by != F.by)
     HM_FAIL(HM_ERR_ARG, "the plan is laid out for tiles of %d x %d, this block takes %d x %d", F.bx, F.by, bx, by);
-  auto kern = k_fused<CT, NT, NC, QP, NQ, OT, BS>;
+  const FusedSizes Z =
+      fu_sizes<CT, NT, NC, NQ>(bx, by, W.nslot, P.nd_u, P.nd_m, SP.need_fields != 0, AL, FU_MAXZ);
+  auto kern = k_fused<CT, NT, NC, QP, NQ, OT, BS, AL>;
   if (Z.bytes > 48 * 1024) {
     const hipError_t err = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                                (int)Z.bytes);
@@ -765,6 +785,20 @@ static int fu_launch(const hm_problem& P, int i, int j, const SumfactPlan& SP, c
   for (int p = 0; p < SF_MAXPAIR; ++p) G.ws[p] = W.ws[p];
   G.nslot = W.nslot;
   G.wfirst = W.wfirst;
+  // the pairs of each third-direction type, ascending, for a general and an aligned element
+  for (int kind = 0; kind < 2; ++kind)
+    for (int t = 0; t < 4; ++t) {
+      int n = 0;
+      G.pn[kind][t] = 0;
+      if (AL && kind == 0) continue;  // every element aligned: the general lists are not read
+      for (int p = 0; p < SP.npair; ++p) {
+        if (!((SP.tmask[t] >> p) & 1) || (kind == 1 && ((SP.offdiag >> p) & 1))) continue;
+        const int d1 = ((SP.pr[p] == 1) << 1) | (SP.pc[p] == 1);
+        const int d2 = ((SP.pr[p] == 2) << 1) | (SP.pc[p] == 2);
+        G.pl[kind][t][n++] = d1 | (d2 << 2) | (W.ws[p] << 4);
+      }
+      G.pn[kind][t] = n;
+    }
   // layers a chunk: the fewest rounds of blocks over the compute units times the layers a
   // block sweeps (one more for the halo layer of a cut column)
   int cus = 256, dev = 0;
@@ -796,17 +830,25 @@ static int fu_launch(const hm_problem& P, int i, int j, const SumfactPlan& SP, c
   return hip_check("k_fused");
 }
 
-template <typename CT, int QP, typename OT>
+template <typename CT, int QP, typename OT, bool AL>
 static int fu_orders(const hm_problem& P, int i, int j, int nt, int nc, const SumfactPlan& SP,
                      const hm_fused_plan& F, void* data, hipStream_t st) {
   if (P.nq1 == 4) {
-    if (nt == 3 && nc == 3) return fu_launch<CT, 3, 3, QP, 4, OT>(P, i, j, SP, F, data, st);
-    if (nt == 3 && nc == 2) return fu_launch<CT, 3, 2, QP, 4, OT>(P, i, j, SP, F, data, st);
-    return fu_launch<CT, 2, 2, QP, 4, OT>(P, i, j, SP, F, data, st);
+    if (nt == 3 && nc == 3) return fu_launch<CT, 3, 3, QP, 4, OT, AL>(P, i, j, SP, F, data, st);
+    if (nt == 3 && nc == 2) return fu_launch<CT, 3, 2, QP, 4, OT, AL>(P, i, j, SP, F, data, st);
+    return fu_launch<CT, 2, 2, QP, 4, OT, AL>(P, i, j, SP, F, data, st);
   }
-  if (nt == 3 && nc == 3) return fu_launch<CT, 3, 3, QP, 3, OT>(P, i, j, SP, F, data, st);
-  if (nt == 3 && nc == 2) return fu_launch<CT, 3, 2, QP, 3, OT>(P, i, j, SP, F, data, st);
-  return fu_launch<CT, 2, 2, QP, 3, OT>(P, i, j, SP, F, data, st);
+  if (nt == 3 && nc == 3) return fu_launch<CT, 3, 3, QP, 3, OT, AL>(P, i, j, SP, F, data, st);
+  if (nt == 3 && nc == 2) return fu_launch<CT, 3, 2, QP, 3, OT, AL>(P, i, j, SP, F, data, st);
+  return fu_launch<CT, 2, 2, QP, 3, OT, AL>(P, i, j, SP, F, data, st);
+}
+
+// the mesh's kind: every element axis-aligned (plan flag), or any
+template <typename CT, int QP, typename OT>
+static int fu_form(const hm_problem& P, int i, int j, int nt, int nc, const SumfactPlan& SP, const hm_fused_plan& F,
+                   void* data, hipStream_t st) {
+  if (F.flags & HM_FUSED_ALIGNED) return fu_orders<CT, QP, OT, true>(P, i, j, nt, nc, SP, F, data, st);
+  return fu_orders<CT, QP, OT, false>(P, i, j, nt, nc, SP, F, data, st);
 }
 
 }  // namespace hm
@@ -821,15 +863,15 @@ extern "C" int hm_fused_tile(const hm_problem* P, int i, int j, int precision, i
   int nt = 0, nc = 0;
   if (int rc = fu_plan(P, i, j, SP, &nt, &nc)) return rc;
   int x = 0, y = 0, rc = HM_OK;
-  const int ns = fu_slots(SP, i, j).nslot;
+  const bool f64 = precision == HM_FP64;
   if (P->nq1 == 4) {
-    rc = nt == 3 && nc == 3 ? fu_tile<3, 3, 4>(P, SP, ns, &x, &y)
-         : nt == 3         ? fu_tile<3, 2, 4>(P, SP, ns, &x, &y)
-                           : fu_tile<2, 2, 4>(P, SP, ns, &x, &y);
+    rc = nt == 3 && nc == 3 ? (f64 ? fu_tile<double, 3, 3, 4>(P, SP, i, j, &x, &y) : fu_tile<float, 3, 3, 4>(P, SP, i, j, &x, &y))
+         : nt == 3          ? (f64 ? fu_tile<double, 3, 2, 4>(P, SP, i, j, &x, &y) : fu_tile<float, 3, 2, 4>(P, SP, i, j, &x, &y))
+                            : (f64 ? fu_tile<double, 2, 2, 4>(P, SP, i, j, &x, &y) : fu_tile<float, 2, 2, 4>(P, SP, i, j, &x, &y));
   } else {
-    rc = nt == 3 && nc == 3 ? fu_tile<3, 3, 3>(P, SP, ns, &x, &y)
-         : nt == 3         ? fu_tile<3, 2, 3>(P, SP, ns, &x, &y)
-                           : fu_tile<2, 2, 3>(P, SP, ns, &x, &y);
+    rc = nt == 3 && nc == 3 ? (f64 ? fu_tile<double, 3, 3, 3>(P, SP, i, j, &x, &y) : fu_tile<float, 3, 3, 3>(P, SP, i, j, &x, &y))
+         : nt == 3          ? (f64 ? fu_tile<double, 3, 2, 3>(P, SP, i, j, &x, &y) : fu_tile<float, 3, 2, 3>(P, SP, i, j, &x, &y))
+                            : (f64 ? fu_tile<double, 2, 2, 3>(P, SP, i, j, &x, &y) : fu_tile<float, 2, 2, 3>(P, SP, i, j, &x, &y));
   }
   *bx = x;
   *by = y;
@@ -847,8 +889,8 @@ extern "C" int hm_assemble_fused(const hm_problem* P, int i, int j, const hm_fus
   if (int rc = fu_plan(P, i, j, SP, &nt, &nc)) return rc;
   hipStream_t st = (hipStream_t)stream;
   if (precision == HM_FP64 && out_dtype == HM_FP64)
-    return fu_orders<double, HM_FP64, double>(*P, i, j, nt, nc, SP, *F, data, st);
+    return fu_form<double, HM_FP64, double>(*P, i, j, nt, nc, SP, *F, data, st);
   if (precision == HM_FP32 && out_dtype == HM_FP32)
-    return fu_orders<float, HM_FP32, float>(*P, i, j, nt, nc, SP, *F, data, st);
+    return fu_form<float, HM_FP32, float>(*P, i, j, nt, nc, SP, *F, data, st);
   HM_FAIL(HM_ERR_UNSUPPORTED, "fused assembly of precision %d into values of type %d", precision, out_dtype);
 }

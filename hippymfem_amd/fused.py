@@ -83,6 +83,29 @@ def _widths(L, p, n, pc):
     return pc * np.where(shared, 2, 1) + 1
 
 
+def aligned_box(mesh):
+    """Whether every element of ``mesh`` is an axis-aligned box: each vertex coordinate equals
+    that of the vertex with the same index along its axis (the test the element kernels make
+    element by element, which then take J as the diagonal of the edge lengths).  Made once a
+    mesh and kept on it."""
+    got = getattr(mesh, "_hm_aligned", None)
+    if got is None:
+        V = np.asarray(mesh.verts)
+        ev = np.asarray(mesh.elem_verts)
+        got = True
+        for lo in range(0, ev.shape[0], 1 << 18):
+            blk = ev[lo:lo + (1 << 18)]
+            for d, bit in ((0, 1), (1, 2), (2, 4)):
+                C = V[blk, d]
+                if not np.array_equal(C, C[:, [v & bit for v in range(8)]]):
+                    got = False
+                    break
+            if not got:
+                break
+        mesh._hm_aligned = got
+    return got
+
+
 def _first(L, p, n):
     """The first element holding each lattice point (the stencil's origin, in elements)."""
     l = np.arange(L)
@@ -199,7 +222,8 @@ class FusedPlan:
         self.chunk_len = chunk                  # (plane, tile) entries before padding
         self.hdr_off_np, self.sig_off_np = hdr_off, sig_off
         self.struct = _lib.HmFusedPlan(
-            nx=nx, ny=ny, nz=nz, bx=bx, by=by, reserved=0, hdr=_lib.ptr(self.hdr),
+            nx=nx, ny=ny, nz=nz, bx=bx, by=by,
+            flags=_lib.FUSED_ALIGNED if aligned_box(mesh) else 0, hdr=_lib.ptr(self.hdr),
             hdr_off=_lib.ptr(self.hdr_off), sigma=_lib.ptr(self.sigma),
             sig_off=_lib.ptr(self.sig_off), diag=0.0 if diag is None else float(diag))
 

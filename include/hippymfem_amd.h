@@ -285,13 +285,20 @@ int hm_csr_matvec(int64_t nrow, const int32_t* indptr, const int32_t* indices,
 typedef struct hm_fused_plan {
   int32_t nx, ny, nz;
   int32_t bx, by;
-  int32_t reserved;
+  int32_t flags;           /* HM_FUSED_ALIGNED: every element an axis-aligned box (J diagonal) */
   const int32_t* hdr;      /* (nrow, 2): CSR offset, row mode; tile order                    */
   const int64_t* hdr_off;  /* (planes * tiles): first row of each (plane, tile) in hdr       */
   const uint8_t* sigma;    /* stencil place of every CSR entry, tile order, bit 7: dropped   */
   const int64_t* sig_off;  /* (planes * tiles): start of each (plane, tile), 16-byte aligned */
   double diag;             /* the diagonal of rows of mode 2                                 */
 } hm_fused_plan;
+
+/* hm_fused_plan.flags: HM_FUSED_ALIGNED when every element of the mesh is an axis-aligned box
+ * (each vertex coordinate equals that of the vertex with the same index along its axis, the test
+ * the element kernels make element by element): no element then needs the pointwise pairs of two
+ * different reference derivatives, and the kernel leaves out the per-element test.  A plan made
+ * for a mesh stays with that mesh's coordinates. */
+#define HM_FUSED_ALIGNED 1
 
 int hm_fused_tile(const hm_problem* P, int i, int j, int precision, int32_t* bx, int32_t* by);
 int hm_assemble_fused(const hm_problem* P, int i, int j, const hm_fused_plan* F, int precision,
