@@ -67,10 +67,11 @@ __global__ void __launch_bounds__(256) k_element_matrices(hm_problem P, int s, i
     __syncthreads();
     load_element(P, e, Xe, ue, me, pe);
     __syncthreads();
+    // (the FP32 form: fields, kappa, the pairs and the contraction in FP32, the geometry in FP64)
     for (int q = threadIdx.x; q < nq; q += blockDim.x) {
-      PointState S;
-      point_state(P, q, Xe, ue, me, pe, S);
-      double Dq[4][4];
+      PointStateT<OutT> S;
+      point_state<OutT>(P, q, Xe, ue, me, pe, S);
+      OutT Dq[4][4];
       point_block(s, j, S, Dq);
       for (int r = 0; r < 4; ++r)
         for (int c = 0; c < 4; ++c) D[q * 16 + r * 4 + c] = Dq[r][c];
@@ -79,9 +80,9 @@ __global__ void __launch_bounds__(256) k_element_matrices(hm_problem P, int s, i
     // T[q][r][b] = sum_c D[q][r][c] Psi_c[q][c][b]
     for (int i = threadIdx.x; i < nq * 4 * nd_c; i += blockDim.x) {
       const int b = i % nd_c, r = (i / nd_c) % 4, q = i / (4 * nd_c);
-      double acc = 0;
+      OutT acc = 0;
       if (r >= r0 && r < r1)
-        for (int c = c0; c < c1; ++c) acc = fma(D[q * 16 + r * 4 + c], Pc[(q * 4 + c) * nd_c + b], acc);
+        for (int c = c0; c < c1; ++c) acc = fma((OutT)D[q * 16 + r * 4 + c], (OutT)Pc[(q * 4 + c) * nd_c + b], acc);
       T[i] = acc;
     }
     __syncthreads();
@@ -89,10 +90,10 @@ __global__ void __launch_bounds__(256) k_element_matrices(hm_problem P, int s, i
     OutT* Eo = out + (size_t)(e - e0) * nd_t * nd_c;
     for (int i = threadIdx.x; i < nd_t * nd_c; i += blockDim.x) {
       const int a = i / nd_c, b = i % nd_c;
-      double acc = 0;
+      OutT acc = 0;
       for (int q = 0; q < nq; ++q)
-        for (int r = r0; r < r1; ++r) acc = fma(Pt[(q * 4 + r) * nd_t + a], T[(q * 4 + r) * nd_c + b], acc);
-      Eo[i] = (OutT)acc;
+        for (int r = r0; r < r1; ++r) acc = fma((OutT)Pt[(q * 4 + r) * nd_t + a], (OutT)T[(q * 4 + r) * nd_c + b], acc);
+      Eo[i] = acc;
     }
   }
 }
@@ -593,8 +594,8 @@ extern "C" int hm_element_matrices(const hm_problem* P, int i, int j, int64_t e0
     if (rc != HM_ERR_UNSUPPORTED) return rc;
     // the matrix-core tables do not fit (many points and parts): the generic contraction
   }
-  // The generic contraction, FP64 arithmetic; FP32 results are its sums rounded once (the
-  // route of the single-precision forms where the others do not apply).
+  // The generic contraction, in FP64, or in FP32 for the FP32 form (the route of the
+  // single-precision forms where the others do not apply).
   const int nq = P->nq;
   const int nd_t = i == HM_PARAMETER ? P->nd_m : P->nd_u;
   const int nd_c = j == HM_PARAMETER ? P->nd_m : P->nd_u;

@@ -444,10 +444,11 @@ __global__ void __launch_bounds__(BS) k_fused(const hm_problem P, int s, int j, 
       const int m = t / NQ3, q = t - m * NQ3;
       const double* D = Dreg(m);
       CT* We = Wreg(m);
-      PointState S;
-      sf_point_state<NQ>(P, q, D, D + 24 + nm, D + 24, D + 24 + nm + nu, fields, TX, TM, TW, T1U, S,
-                         AL || EFLAG[m] != 0);
-      double Db[4][4];
+      // (the FP32 form makes the fields, kappa and the pairs in FP32, the geometry in FP64)
+      PointStateT<CT> S;
+      sf_point_state<NQ, CT>(P, q, D, D + 24 + nm, D + 24, D + 24 + nm + nu, fields, TX, TM, TW, T1U, S,
+                             AL || EFLAG[m] != 0);
+      CT Db[4][4];
       point_block(s, j, S, Db);
       int p = 0;
 #pragma unroll

@@ -107,9 +107,10 @@ __global__ void __launch_bounds__(256) k_element_mfma(hm_problem P, int s, int j
     __syncthreads();
     if (active) {
       for (int q = lane; q < nq; q += 64) {
-        PointState st;
-        point_state(P, q, Xe, ue, me, pe, st);
-        double Dq[4][4];
+        // (the FP32 form: fields, kappa and the pairs in FP32, the geometry in FP64)
+        PointStateT<VT> st;
+        point_state<VT>(P, q, Xe, ue, me, pe, st);
+        VT Dq[4][4];
         point_block(s, j, st, Dq);
         for (int r = 0; r < 4; ++r)
           for (int c = 0; c < 4; ++c) D[q * 16 + r * 4 + c] = Dq[r][c];
@@ -120,14 +121,14 @@ __global__ void __launch_bounds__(256) k_element_mfma(hm_problem P, int s, int j
       // T[K][b] = sum_c D[q][r][c] Psi_c[q][c][b]
       for (int i = lane; i < Kp * NW; i += 64) {
         const int K = i / NW, b = i % NW;
-        double v = 0.0;
+        VT v = 0;
         if (K < Kv) {
           const int q = K / S.nrt, r = S.rt0 + K % S.nrt;
           const double* Dr = D + q * 16 + r * 4 + S.ct0;
           const double* pc = Pc + (size_t)q * S.nct * NW + b;
-          for (int ci = 0; ci < S.nct; ++ci) v = fma(Dr[ci], pc[ci * NW], v);
+          for (int ci = 0; ci < S.nct; ++ci) v = fma((VT)Dr[ci], (VT)pc[ci * NW], v);
         }
-        T[i] = (VT)v;
+        T[i] = v;
       }
     }
     __syncthreads();

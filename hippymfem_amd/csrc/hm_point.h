@@ -16,36 +16,41 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
-// Everything one quadrature point of one element needs, in FP64.
-struct PointState {
-  double w;           // w_q |det J|
-  double Ji[3][3];    // Jinv[k][d] = dxi_k / dx_d
-  double kap, g, g1, g2;
-  double gu[3], gp[3];  // physical gradients of u and p
-  double dot;           // grad u . grad p
-  double uq;            // value of u
+// Everything one quadrature point of one element needs, in the point type PT: FP64, or FP32
+// in the precision forms (whose geometry is made in FP64 and rounded once).
+template <typename PT>
+struct PointStateT {
+  PT w;           // w_q |det J|
+  PT Ji[3][3];    // Jinv[k][d] = dxi_k / dx_d
+  PT kap, g, g1, g2;
+  PT gu[3], gp[3];  // physical gradients of u and p
+  PT dot;           // grad u . grad p
+  PT uq;            // value of u
 };
+using PointState = PointStateT<double>;
 
 // The point state from the Jacobian of the element map J[d][k] = dx_d / dxi_k, the point's
-// quadrature weight and the fields' values and reference gradients at the point.
-__device__ inline void point_finish(const hm_problem& P, double wq, const double J[3][3], double uq,
-                                    const double ru[3], const double rp[3], double mq, PointState& S) {
+// quadrature weight and the fields' values and reference gradients at the point.  The
+// determinant and the inverse of J are FP64 whatever PT is; everything after them is PT.
+template <typename PT>
+__device__ inline void point_finish(const hm_problem& P, double wq, const double J[3][3], PT uq, const PT ru[3],
+                                    const PT rp[3], PT mq, PointStateT<PT>& S) {
   const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
   const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
   const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
   const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
   const double id = 1.0 / det;
   // inverse of J (rows k = reference direction, columns d = physical)
-  S.Ji[0][0] = c00 * id;
-  S.Ji[1][0] = c01 * id;
-  S.Ji[2][0] = c02 * id;
-  S.Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
-  S.Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
-  S.Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
-  S.Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
-  S.Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
-  S.Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
-  S.w = wq * fabs(det);
+  S.Ji[0][0] = (PT)(c00 * id);
+  S.Ji[1][0] = (PT)(c01 * id);
+  S.Ji[2][0] = (PT)(c02 * id);
+  S.Ji[0][1] = (PT)((J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id);
+  S.Ji[1][1] = (PT)((J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id);
+  S.Ji[2][1] = (PT)((J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id);
+  S.Ji[0][2] = (PT)((J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id);
+  S.Ji[1][2] = (PT)((J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id);
+  S.Ji[2][2] = (PT)((J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id);
+  S.w = (PT)(wq * fabs(det));
   for (int d = 0; d < 3; ++d) {
     S.gu[d] = S.Ji[0][d] * ru[0] + S.Ji[1][d] * ru[1] + S.Ji[2][d] * ru[2];
     S.gp[d] = S.Ji[0][d] * rp[0] + S.Ji[1][d] * rp[1] + S.Ji[2][d] * rp[2];
@@ -54,21 +59,23 @@ __device__ inline void point_finish(const hm_problem& P, double wq, const double
   S.kap = exp(mq);
   S.uq = uq;
   if (P.density == HM_DENSITY_NONLINEAR) {
-    S.g = 1.0 + uq * uq;
-    S.g1 = 2.0 * uq;
-    S.g2 = 2.0;
+    S.g = (PT)1 + uq * uq;
+    S.g1 = (PT)2 * uq;
+    S.g2 = (PT)2;
   } else {
-    S.g = 1.0;
-    S.g1 = 0.0;
-    S.g2 = 0.0;
+    S.g = (PT)1;
+    S.g1 = (PT)0;
+    S.g2 = (PT)0;
   }
 }
 
 // tab_x / tab_m / tab_u may point at LDS copies of the problem's tables (the default: the
-// problem's own, in global memory).
+// problem's own, in global memory).  PT as point_finish: the fields in PT (dof values and tables
+// rounded as read), the geometry in FP64.
+template <typename PT>
 __device__ inline void point_state(const hm_problem& P, int q, const double* Xe,
                                    const double* ue, const double* me, const double* pe,
-                                   PointState& S, bool fields = true,
+                                   PointStateT<PT>& S, bool fields = true,
                                    const double* tab_x = nullptr, const double* tab_m = nullptr,
                                    const double* tab_u = nullptr) {
   const int nq = P.nq;
@@ -86,17 +93,17 @@ __device__ inline void point_state(const hm_problem& P, int q, const double* Xe,
   const double* Nu = tu + (size_t)q * nu;
   const double* Gu = tu + (size_t)nq * nu + (size_t)q * nu * 3;
   const double* Nm = (tab_m ? tab_m : P.tab_m) + (size_t)q * nm;
-  double uq = 0, mq = 0, ru[3] = {0, 0, 0}, rp[3] = {0, 0, 0};
+  PT uq = 0, mq = 0, ru[3] = {0, 0, 0}, rp[3] = {0, 0, 0};
 #pragma unroll 2
   for (int a = 0; a < (fields ? nu : 0); ++a) {
-    uq = fma(Nu[a], ue[a], uq);
+    uq = fma((PT)Nu[a], (PT)ue[a], uq);
     for (int k = 0; k < 3; ++k) {
-      ru[k] = fma(Gu[a * 3 + k], ue[a], ru[k]);
-      rp[k] = fma(Gu[a * 3 + k], pe[a], rp[k]);
+      ru[k] = fma((PT)Gu[a * 3 + k], (PT)ue[a], ru[k]);
+      rp[k] = fma((PT)Gu[a * 3 + k], (PT)pe[a], rp[k]);
     }
   }
-  for (int a = 0; a < nm; ++a) mq = fma(Nm[a], me[a], mq);
-  point_finish(P, P.wq[q], J, uq, ru, rp, mq, S);
+  for (int a = 0; a < nm; ++a) mq = fma((PT)Nm[a], (PT)me[a], mq);
+  point_finish<PT>(P, P.wq[q], J, uq, ru, rp, mq, S);
 }
 
 // The point state of element e at point q with every operand read from global memory as it
@@ -138,9 +145,10 @@ __device__ inline void point_state_stream(const hm_problem& P, int64_t e, int q,
 // The pointwise block of test slot s and trial slot j, in the reference-gradient basis,
 // D[r][c] with r, c in {value, d/dxi_0, d/dxi_1, d/dxi_2}, times w.  The closed forms of
 // the density r = exp(m) g(u) grad u . grad p - f p (second derivatives per part).
-__device__ inline void point_block(int s, int j, const PointState& S, double D[4][4]) {
-  double vv = 0, vg[3] = {0, 0, 0}, gv[3] = {0, 0, 0}, gg = 0;
-  const double kg = S.kap * S.g, kg1 = S.kap * S.g1;
+template <typename PT>
+__device__ inline void point_block(int s, int j, const PointStateT<PT>& S, PT D[4][4]) {
+  PT vv = 0, vg[3] = {0, 0, 0}, gv[3] = {0, 0, 0}, gg = 0;
+  const PT kg = S.kap * S.g, kg1 = S.kap * S.g1;
   const int key = s * 3 + j;
   switch (key) {
     case HM_STATE * 3 + HM_STATE:
@@ -175,7 +183,7 @@ __device__ inline void point_block(int s, int j, const PointState& S, double D[4
     default:
       break;
   }
-  const double w = S.w;
+  const PT w = S.w;
   D[0][0] = w * vv;
   for (int k = 0; k < 3; ++k) {
     D[0][1 + k] = w * (S.Ji[k][0] * vg[0] + S.Ji[k][1] * vg[1] + S.Ji[k][2] * vg[2]);
@@ -183,7 +191,7 @@ __device__ inline void point_block(int s, int j, const PointState& S, double D[4
   }
   for (int k = 0; k < 3; ++k)
     for (int l = 0; l < 3; ++l) {
-      const double K = S.Ji[k][0] * S.Ji[l][0] + S.Ji[k][1] * S.Ji[l][1] + S.Ji[k][2] * S.Ji[l][2];
+      const PT K = S.Ji[k][0] * S.Ji[l][0] + S.Ji[k][1] * S.Ji[l][1] + S.Ji[k][2] * S.Ji[l][2];
       D[1 + k][1 + l] = w * gg * K;
     }
 }

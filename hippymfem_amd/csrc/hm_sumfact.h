@@ -73,10 +73,14 @@ __host__ __device__ inline int sf_element_words(int npair, int nd_u, int nd_m, i
 // The fields u, p and their reference gradients are summed one direction at a time: for each
 // (a2, a3) the sums over a1 of u and p against the value and the derivative of the 1-D basis
 // at q1, then those times the 1-D factors at q2 and q3.
-template <int NQ>
+//
+// PT: the point type, FP64, or FP32 in the precision forms: the fields, their gradients and
+// everything after the geometry are then FP32 (the dof values and tables rounded as read), the
+// geometry FP64 (point_finish).
+template <int NQ, typename PT = double>
 __device__ inline void sf_point_state(const hm_problem& P, int q, const double* X, const double* U,
                                       const double* M, const double* Pv, bool fields, const double* TX,
-                                      const double* TM, const double* TW, const double* T1U, PointState& S,
+                                      const double* TM, const double* TW, const double* T1U, PointStateT<PT>& S,
                                       bool aligned = false) {
   double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
   // aligned: the element is an axis-aligned box (every vertex coordinate that of the vertex with
@@ -100,7 +104,7 @@ __device__ inline void sf_point_state(const hm_problem& P, int q, const double* 
       }
   }
   const int nm = P.nd_m;
-  double uq = 0, mq = 0, ru[3] = {0, 0, 0}, rp[3] = {0, 0, 0};
+  PT uq = 0, mq = 0, ru[3] = {0, 0, 0}, rp[3] = {0, 0, 0};
   if (fields) {
     const int n1 = P.n1_u;
     const int q1 = q % NQ, q2 = (q / NQ) % NQ, q3 = q / (NQ * NQ);
@@ -108,23 +112,23 @@ __device__ inline void sf_point_state(const hm_problem& P, int q, const double* 
     // shares the kernel's register budget with the contraction)
 #pragma unroll 1
     for (int a3 = 0; a3 < n1; ++a3) {
-      const double l3 = T1U[a3 * NQ + q3], d3 = T1U[(n1 + a3) * NQ + q3];
+      const PT l3 = (PT)T1U[a3 * NQ + q3], d3 = (PT)T1U[(n1 + a3) * NQ + q3];
 #pragma unroll 1
       for (int a2 = 0; a2 < n1; ++a2) {
-        const double l2 = T1U[a2 * NQ + q2], d2 = T1U[(n1 + a2) * NQ + q2];
+        const PT l2 = (PT)T1U[a2 * NQ + q2], d2 = (PT)T1U[(n1 + a2) * NQ + q2];
         const double* u = U + n1 * (a2 + n1 * a3);
         const double* pv = Pv + n1 * (a2 + n1 * a3);
-        double svu = 0, sdu = 0, svp = 0, sdp = 0;
+        PT svu = 0, sdu = 0, svp = 0, sdp = 0;
 #pragma unroll
         for (int a1 = 0; a1 < 3; ++a1) {
           if (a1 >= n1) break;
-          const double l1 = T1U[a1 * NQ + q1], d1 = T1U[(n1 + a1) * NQ + q1];
-          svu = fma(l1, u[a1], svu);
-          sdu = fma(d1, u[a1], sdu);
-          svp = fma(l1, pv[a1], svp);
-          sdp = fma(d1, pv[a1], sdp);
+          const PT l1 = (PT)T1U[a1 * NQ + q1], d1 = (PT)T1U[(n1 + a1) * NQ + q1];
+          svu = fma(l1, (PT)u[a1], svu);
+          sdu = fma(d1, (PT)u[a1], sdu);
+          svp = fma(l1, (PT)pv[a1], svp);
+          sdp = fma(d1, (PT)pv[a1], sdp);
         }
-        const double f0 = l2 * l3, f1 = d2 * l3, f2 = l2 * d3;
+        const PT f0 = l2 * l3, f1 = d2 * l3, f2 = l2 * d3;
         uq = fma(f0, svu, uq);
         ru[0] = fma(f0, sdu, ru[0]);
         ru[1] = fma(f1, svu, ru[1]);
@@ -135,8 +139,8 @@ __device__ inline void sf_point_state(const hm_problem& P, int q, const double* 
       }
     }
   }
-  for (int a = 0; a < nm; ++a) mq = fma(TM[a * (NQ * NQ * NQ) + q], M[a], mq);
-  point_finish(P, TW[q], J, uq, ru, rp, mq, S);
+  for (int a = 0; a < nm; ++a) mq = fma((PT)TM[a * (NQ * NQ * NQ) + q], (PT)M[a], mq);
+  point_finish<PT>(P, TW[q], J, uq, ru, rp, mq, S);
 }
 
 // LDS words of the tables every element reads (nq points, nq1 a direction): the trilinear

@@ -20,6 +20,8 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "hm_common.h"
 #include "hm_device.h"
 #include "hm_point.h"
@@ -241,11 +243,13 @@ __attribute__((amdgpu_waves_per_eu(sf_waves<CT, NT, NC, NQ>(), sf_waves<CT, NT, 
     fetch_ids(el + stride);
     wave_sync();
     // 2. lane = quadrature point: the pointwise pairs into LDS, in the order of k = 4 r + c
+    // (the precision forms make the fields, kappa and the pairs in FP32, the geometry in FP64)
     auto point_task = [&](int q, double* We, const double* Xe, const double* Ue, const double* Me,
                           const double* Pe, bool al) {
-      PointState S;
-      sf_point_state<NQ>(P, q, Xe, Ue, Me, Pe, fields, TX, TM, TW, T1U, S, al);
-      double D[4][4];
+      using PT = typename std::conditional<QP == HM_FP64, double, float>::type;
+      PointStateT<PT> S;
+      sf_point_state<NQ, PT>(P, q, Xe, Ue, Me, Pe, fields, TX, TM, TW, T1U, S, al);
+      PT D[4][4];
       point_block(s, j, S, D);
       int p = 0;
 #pragma unroll
