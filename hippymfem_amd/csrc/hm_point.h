@@ -22,6 +22,7 @@ template <typename PT>
 struct PointStateT {
   PT w;           // w_q |det J|
   PT Ji[3][3];    // Jinv[k][d] = dxi_k / dx_d
+  PT K[3][3];     // sum_d Jinv[k][d] Jinv[l][d], the weights of the reference-gradient pairs
   PT kap, g, g1, g2;
   PT gu[3], gp[3];  // physical gradients of u and p
   PT dot;           // grad u . grad p
@@ -29,28 +30,51 @@ struct PointStateT {
 };
 using PointState = PointStateT<double>;
 
-// The point state from the Jacobian of the element map J[d][k] = dx_d / dxi_k, the point's
-// quadrature weight and the fields' values and reference gradients at the point.  The
-// determinant and the inverse of J are FP64 whatever PT is; everything after them is PT.
+// The geometry of the element map at a point (of the whole element where the map is affine):
+// |det J| in FP64, J^-1 and the pair weights K in PT, each rounded once from FP64.
 template <typename PT>
-__device__ inline void point_finish(const hm_problem& P, double wq, const double J[3][3], PT uq, const PT ru[3],
-                                    const PT rp[3], PT mq, PointStateT<PT>& S) {
+struct GeomT {
+  double adet;
+  PT Ji[3][3];
+  PT K[3][3];
+};
+
+// The geometry from the Jacobian of the element map J[d][k] = dx_d / dxi_k: its determinant and
+// inverse in FP64, whatever PT is.
+template <typename PT>
+__device__ inline void geom_from_J(const double J[3][3], GeomT<PT>& G) {
   const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
   const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
   const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
   const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
   const double id = 1.0 / det;
   // inverse of J (rows k = reference direction, columns d = physical)
-  S.Ji[0][0] = (PT)(c00 * id);
-  S.Ji[1][0] = (PT)(c01 * id);
-  S.Ji[2][0] = (PT)(c02 * id);
-  S.Ji[0][1] = (PT)((J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id);
-  S.Ji[1][1] = (PT)((J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id);
-  S.Ji[2][1] = (PT)((J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id);
-  S.Ji[0][2] = (PT)((J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id);
-  S.Ji[1][2] = (PT)((J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id);
-  S.Ji[2][2] = (PT)((J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id);
-  S.w = (PT)(wq * fabs(det));
+  G.Ji[0][0] = (PT)(c00 * id);
+  G.Ji[1][0] = (PT)(c01 * id);
+  G.Ji[2][0] = (PT)(c02 * id);
+  G.Ji[0][1] = (PT)((J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id);
+  G.Ji[1][1] = (PT)((J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id);
+  G.Ji[2][1] = (PT)((J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id);
+  G.Ji[0][2] = (PT)((J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id);
+  G.Ji[1][2] = (PT)((J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id);
+  G.Ji[2][2] = (PT)((J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id);
+  G.adet = fabs(det);
+  for (int k = 0; k < 3; ++k)
+    for (int l = 0; l < 3; ++l)
+      G.K[k][l] = G.Ji[k][0] * G.Ji[l][0] + G.Ji[k][1] * G.Ji[l][1] + G.Ji[k][2] * G.Ji[l][2];
+}
+
+// The point state from the geometry at the point, the point's quadrature weight and the
+// fields' values and reference gradients there, all after the geometry in PT.
+template <typename PT>
+__device__ inline void point_fields(const hm_problem& P, double wq, const GeomT<PT>& G, PT uq, const PT ru[3],
+                                    const PT rp[3], PT mq, PointStateT<PT>& S) {
+  for (int k = 0; k < 3; ++k)
+    for (int l = 0; l < 3; ++l) {
+      S.Ji[k][l] = G.Ji[k][l];
+      S.K[k][l] = G.K[k][l];
+    }
+  S.w = (PT)(wq * G.adet);
   for (int d = 0; d < 3; ++d) {
     S.gu[d] = S.Ji[0][d] * ru[0] + S.Ji[1][d] * ru[1] + S.Ji[2][d] * ru[2];
     S.gp[d] = S.Ji[0][d] * rp[0] + S.Ji[1][d] * rp[1] + S.Ji[2][d] * rp[2];
@@ -67,6 +91,15 @@ __device__ inline void point_finish(const hm_problem& P, double wq, const double
     S.g1 = (PT)0;
     S.g2 = (PT)0;
   }
+}
+
+// The point state from the Jacobian of the element map at the point.
+template <typename PT>
+__device__ inline void point_finish(const hm_problem& P, double wq, const double J[3][3], PT uq, const PT ru[3],
+                                    const PT rp[3], PT mq, PointStateT<PT>& S) {
+  GeomT<PT> G;
+  geom_from_J<PT>(J, G);
+  point_fields<PT>(P, wq, G, uq, ru, rp, mq, S);
 }
 
 // tab_x / tab_m / tab_u may point at LDS copies of the problem's tables (the default: the
@@ -190,10 +223,7 @@ __device__ inline void point_block(int s, int j, const PointStateT<PT>& S, PT D[
     D[1 + k][0] = w * (S.Ji[k][0] * gv[0] + S.Ji[k][1] * gv[1] + S.Ji[k][2] * gv[2]);
   }
   for (int k = 0; k < 3; ++k)
-    for (int l = 0; l < 3; ++l) {
-      const PT K = S.Ji[k][0] * S.Ji[l][0] + S.Ji[k][1] * S.Ji[l][1] + S.Ji[k][2] * S.Ji[l][2];
-      D[1 + k][1 + l] = w * gg * K;
-    }
+    for (int l = 0; l < 3; ++l) D[1 + k][1 + l] = w * gg * S.K[k][l];
 }
 
 // Which parts of the (s, j) block can be nonzero: bit 0 the value row, bit 1 the gradient

@@ -234,3 +234,19 @@ def test_fused_plan_refuses_a_mesh_that_is_not_its_box():
         n = 2                                              # a box of the wrong size
     with pytest.raises((NotImplementedError, IndexError)):
         FusedPlan(p.pattern, W.Vu, W.Vu, Skewed(), (2, 2), device="cpu")
+
+
+def test_aligned_box_flag():
+    """``aligned_box``: every element's vertex coordinates those of its axis neighbours (the
+    element kernels' per-element test), True on the unit-cube box, False once interior vertices
+    move, and kept on the mesh."""
+    from hippymfem_amd.fused import aligned_box
+    from hippymfem_amd.mesh import BoxMesh
+    assert aligned_box(BoxMesh(5))
+    moved = BoxMesh(5, perturb=0.2)
+    assert not aligned_box(moved)
+    assert moved._hm_aligned is False
+    one = BoxMesh(3)
+    one.verts = one.verts.copy()
+    one.verts[one.elem_verts[13, 7], 1] += 1e-12          # one interior vertex, one coordinate
+    assert not aligned_box(one)

@@ -104,3 +104,31 @@ def test_fused_nonfinite_raises(gpu):
     with pytest.raises(RuntimeError):
         prob.block(ADJOINT, STATE, [W.u, m, W.p])
     prob.block(ADJOINT, STATE, [W.u, W.m, W.p])            # finite again
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_aligned_form_equals_general_form(gpu, precision):
+    """On a mesh of axis-aligned elements the plan carries HM_FUSED_ALIGNED and the kernel takes
+    its aligned form (no per-element alignment test, slots for the diagonal reference-gradient
+    pairs only); its values are the general form's bits, every block, nonlinear density."""
+    W = Workload(5, nonlinear=True)
+    prob = PDEVariationalProblem(W.spaces, W.density, W.batches, W.bc, is_fwd_linear=False)
+    loc = [torch.as_tensor(v).cuda() for v in (W.u, W.m, W.p)]
+    for ij in BLOCKS + [(STATE, STATE)]:
+        f = prob.block_assembly(ij[0], ij[1], precision, route="fused")
+        plan = f.fused_plan()
+        assert plan is not None and plan.struct.flags == 1, ij
+        A = f.assemble(loc).data.clone()
+        plan.struct.flags = 0
+        try:
+            B = f.assemble(loc).data.clone()
+        finally:
+            plan.struct.flags = 1
+        assert torch.equal(A, B), (ij, precision)
+
+
+def test_perturbed_mesh_takes_general_form(gpu):
+    W = Workload(4, perturb=0.2)
+    prob = PDEVariationalProblem(W.spaces, W.density, W.batches, W.bc)
+    f = prob.block_assembly(ADJOINT, STATE, route="fused")
+    assert f.fused_plan().struct.flags == 0
