@@ -64,6 +64,16 @@ def _errors(d, ref):
             "fro_rel": float(torch.linalg.vector_norm(diff) / torch.linalg.vector_norm(ref))}
 
 
+def _drop(prob, ij, prec):
+    """Free the cached assembly of block ij in precision prec (every route and option)."""
+    for key in [k for k in prob._blocks if k[:3] == (ij[0], ij[1], prec)]:
+        del prob._blocks[key]
+
+
+def _route(blk):
+    return "fused" if blk.fused_plan() is not None else "two-pass"
+
+
 def _blocks(prob, loc, blocks, precisions, steps, n):
     out = []
     for ij in blocks:
@@ -72,15 +82,15 @@ def _blocks(prob, loc, blocks, precisions, steps, n):
         ref = ref_blk.data.clone()
         nnz = ref_blk.pattern.nnz
         line = {"n": n, "block": NAMES[ij], "nnz": nnz, "fp64_ms": t64,
-                "fp64_nnz_s": nnz / (t64 * 1e-3)}
-        del prob._blocks[(ij[0], ij[1], "fp64", None)]
+                "fp64_nnz_s": nnz / (t64 * 1e-3), "fp64_route": _route(ref_blk)}
+        _drop(prob, ij, "fp64")
         del ref_blk
         for prec in precisions:
             blk = prob.block_assembly(*ij, precision=prec)
             t = _time(lambda: blk.assemble(loc), steps)
             line[prec] = dict(ms=t, nnz_s=nnz / (t * 1e-3), speedup_vs_fp64=t64 / t,
-                              **_errors(blk.data, ref))
-            del prob._blocks[(ij[0], ij[1], prec, None)]
+                              route=_route(blk), **_errors(blk.data, ref))
+            _drop(prob, ij, prec)
             del blk
         del ref
         torch.cuda.empty_cache()
