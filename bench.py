@@ -242,9 +242,10 @@ def measured_traffic():
 
 
 def fp32_form(prob, loc, blk64, W, args, stream):
-    """The precision form: FP32 contraction and element matrices, FP32 values (each sum in
-    FP64, rounded once); geometry and pointwise blocks in FP64.  Error against the FP64
-    values of the same step."""
+    """The precision form: the Jacobian, its determinant and inverse in FP64; the fields, the
+    pointwise pairs, the contraction and the values in FP32 (the fused route, two blocks a
+    compute unit on this box mesh, sums the FP32 entries in FP32 in ascending element order;
+    the two-pass route sums them in FP64).  Error against the FP64 values of the same step."""
     import torch
 
     from hippymfem_amd import ADJOINT, STATE

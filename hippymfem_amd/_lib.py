@@ -30,7 +30,7 @@ EXPORTS = ("hm_last_error", "hm_version", "hm_element_matrices", "hm_element_mat
            "hm_pattern_place", "hm_pattern_repeats", "hm_pattern_work_bytes",
            "hm_pattern_build_device", "hm_pattern_place_device", "hm_scatter_matrix",
            "hm_scatter_matrix_defect", "hm_scatter_vector", "hm_csr_matvec", "hm_fused_tile",
-           "hm_assemble_fused", "hm_serving_normal", "hm_serving_field",
+           "hm_fused_blocks", "hm_assemble_fused", "hm_serving_normal", "hm_serving_field",
            "hm_serving_field_noalloc")
 
 P = ctypes.c_void_p
@@ -115,6 +115,7 @@ def lib():
                                              ctypes.c_int, P]),
         "hm_csr_matvec": (ctypes.c_int, [I64, P, P, P, ctypes.c_int, P, P, P]),
         "hm_fused_tile": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P]),
+        "hm_fused_blocks": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, I32, P]),
         "hm_assemble_fused": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, P, ctypes.c_int, P,
                                              ctypes.c_int, P]),
         "hm_serving_normal": (ctypes.c_int, [P, I64, I64, P, P]),

@@ -19,9 +19,10 @@
 //      local y >= 1 before y = 0 and of local x >= 1 before x = 0.  Two elements of one sub-step
 //      never share a row (rows of one parity in x come from one local x), so the adds of a sub-step
 //      are plain read-add-writes without atomics, and the sub-steps are ordered by block barriers.
-//      Every CSR value is so the sum of its entries in ascending element order, from 0: the order
-//      of np.bincount (hippymfem/fem/pattern.py:788), and the bits of the two-pass scatter of the
-//      same element matrices,
+//      Every CSR value is so the sum of its entries in ascending element order, from 0: in FP64
+//      the order and the bits of np.bincount (hippymfem/fem/pattern.py:788) and of the two-pass
+//      scatter of the same element matrices; the FP32 form sums its FP32 entries in FP32 in the
+//      same order (the two-pass scatter sums them in FP64),
 //   5. writes a plane out once no later layer adds to it (after a3 = 0 and a3 = 1; the plane of
 //      a3 = p_t is carried into the next layer) as whole CSR rows, and clears its buffer.
 // A row buffer holds the row's columns as a dense stencil (the trial-lattice box the row couples
@@ -32,6 +33,9 @@
 // The element matrices, the pointwise pairs and the stencil rows never leave the compute unit.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <type_traits>
+
 #include "hm_common.h"
 #include "hm_device.h"
 #include "hm_point.h"
@@ -41,7 +45,7 @@ namespace hm {
 
 constexpr int FU_MAXL = 17;     // lattice points of a tile a direction (p_t bx + 1)
 constexpr int FU_MAXM = 64;     // element slots of a layer
-constexpr int FU_MAXT = 1024;   // contraction tasks of a layer
+constexpr int FU_MAXT = 512;    // contraction tasks of a layer
 constexpr int FU_ITEMS = 2;     // element-data loads a thread a layer
 constexpr int FU_SIGV = 2;      // 16-byte stencil-map loads a thread a layer
 constexpr int FU_SLOTS = 2;     // planes of staged tables
@@ -71,6 +75,12 @@ struct FusedGeom {
   int pl[2][4][SF_MAXPAIR];
   int pn[2][4];
 };
+
+// The type of the plane buffers' sums: FP64 for FP64 values (the bits of np.bincount); the
+// FP32 form sums in FP32, in the same ascending element order, so that its buffers take half
+// the LDS and two blocks share a compute unit.
+template <typename CT> struct FuSum { using T = double; };
+template <> struct FuSum<float> { using T = float; };
 
 // Sizes of a tile's LDS, for the launch and the host's choice of tile.
 struct FusedSizes {
@@ -102,22 +112,26 @@ __host__ inline FusedSizes fu_sizes(int bx, int by, int nslot, int nd_u, int nd_
                   4 * NT * NC * NQ;
   z.elem_words = (size_t)z.m_max * z.ew;
   const size_t s2d = (size_t)z.sx_max * z.sy_max;
-  // even word counts, so that the staged maps after the planes start 16-byte aligned
-  z.plane0_words = (s2d * (2 * pc + 1) + 1) & ~(size_t)1;
-  z.plane1_words = (s2d * (pt == 2 ? pc + 1 : 2 * pc + 1) + 1) & ~(size_t)1;
+  // plane buffers in words of their sum type, 16 bytes whole, so that the staged maps after
+  // them start 16-byte aligned
+  using BT = typename FuSum<CT>::T;
+  constexpr size_t A16 = 16 / sizeof(BT);
+  z.plane0_words = (s2d * (2 * pc + 1) + A16 - 1) & ~(A16 - 1);
+  z.plane1_words = (s2d * (pt == 2 ? pc + 1 : 2 * pc + 1) + A16 - 1) & ~(A16 - 1);
   z.sig_bytes = ((s2d * (2 * pc + 1) + 15) & ~(size_t)15);
-  z.bytes = (z.table_words + z.elem_words + z.plane0_words + z.plane1_words) * sizeof(double) +
+  z.bytes = (z.table_words + z.elem_words) * sizeof(double) + (z.plane0_words + z.plane1_words) * sizeof(BT) +
             FU_SLOTS * z.sig_bytes + FU_SLOTS * (size_t)z.rows_max * sizeof(int2) + (size_t)(pt * nz_max + 1) * 8;
   return z;
 }
 
-template <typename CT, int NT, int NC, int QP, int NQ, typename OT, int BS, bool AL>
-__global__ void __launch_bounds__(BS) k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G,
-                                              FusedSizes Z, const double* __restrict__ t1t,
-                                              const double* __restrict__ t1c, OT* __restrict__ data) {
+// BS threads a block, NB blocks a compute unit (the register budget: NB BS / 256 waves a SIMD).
+template <typename CT, int NT, int NC, int QP, int NQ, typename OT, int BS, int NB, bool AL>
+__global__ void __attribute__((amdgpu_flat_work_group_size(1, BS), amdgpu_waves_per_eu((NB * BS + 255) / 256)))
+k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSizes Z, const double* __restrict__ t1t,
+        const double* __restrict__ t1c, OT* __restrict__ data) {
   constexpr int pt = NT - 1, pc = NC - 1;
   constexpr int NQ2 = NQ * NQ, NQ3 = NQ2 * NQ;
-  static_assert(8 * FU_MAXM <= BS, "eight lanes an element slot");
+  using BT = typename FuSum<CT>::T;
   extern __shared__ double lds[];
   __shared__ int32_t UT[32];
   __shared__ int32_t TASK[FU_MAXT];
@@ -164,8 +178,8 @@ __global__ void __launch_bounds__(BS) k_fused(const hm_problem P, int s, int j, 
   double* TX = lds + sf_table_words(nu, nm, fields, NQ) - NQ3 * 24;
   double* PT = AL ? TX : TX + NQ3 * 24;
   double* EL = lds + Z.table_words;                  // element regions
-  double* BUF0 = EL + Z.elem_words;                  // plane buffers
-  double* BUF1 = BUF0 + Z.plane0_words;
+  BT* BUF0 = (BT*)(EL + Z.elem_words);               // plane buffers
+  BT* BUF1 = BUF0 + Z.plane0_words;
   uint8_t* SIG = (uint8_t*)(BUF1 + Z.plane1_words);  // staged stencil maps, FU_SLOTS planes
   int2* HDR = (int2*)(SIG + FU_SLOTS * Z.sig_bytes); // staged row headers, FU_SLOTS planes
   int64_t* SOFF = (int64_t*)(HDR + FU_SLOTS * Z.rows_max);  // the chunk's planes' map starts
@@ -186,7 +200,7 @@ __global__ void __launch_bounds__(BS) k_fused(const hm_problem P, int s, int j, 
       for (int i = tid; i < 2 * P.n1_u * NQ; i += BS) T1U[i] = P.tab1_u[i];
       for (int i = tid; i < nu; i += BS) UT[P.lex_u[i]] = i;
     }
-    for (size_t i = tid; i < Z.plane0_words + Z.plane1_words; i += BS) BUF0[i] = 0.0;
+    for (size_t i = tid; i < Z.plane0_words + Z.plane1_words; i += BS) BUF0[i] = 0;
     for (int i = tid; i < lz_hi - lz_lo; i += BS) SOFF[i] = G.sig_off[(int64_t)(lz_lo + i) * ntile + tile];
     // stencil widths of the tile's lattice points, and every row's place in a plane buffer
     for (int i = tid; i < cxl; i += BS) WXs[i] = fu_width(lx_lo + i, pt, G.nx, pc);
@@ -373,14 +387,14 @@ __global__ void __launch_bounds__(BS) k_fused(const hm_problem P, int s, int j, 
     const int wz = wz_of(lz);
     const int S = s2d_row * wz;
     const uint32_t mul = (65536u + (uint32_t)wz - 1u) / (uint32_t)wz;
-    double* buf = (lz & 1) ? BUF1 : BUF0;
+    BT* buf = (lz & 1) ? BUF1 : BUF0;
     const uint8_t* sg = SIG + k * Z.sig_bytes;
     const int2* hdr = HDR + k * Z.rows_max;
     for (int base = 0; base < S; base += FU_EB * BS) {
       int kk[FU_EB], rb[FU_EB], rho[FU_EB];
       unsigned c[FU_EB];
       int2 hd[FU_EB];
-      double v[FU_EB];
+      BT v[FU_EB];
       // every read unconditional (a place past the plane reads the last one, and is not
       // written), so that each of the three rounds of reads is issued whole before its first use
 #pragma unroll
@@ -400,9 +414,9 @@ __global__ void __launch_bounds__(BS) k_fused(const hm_problem P, int s, int j, 
 #pragma unroll
       for (int e = 0; e < FU_EB; ++e)
         if (kk[e] < S) {
-          buf[rb[e] + (c[e] & 0x7F)] = 0.0;
+          buf[rb[e] + (c[e] & 0x7F)] = 0;
           const int t = kk[e] - rb[e];
-          double o = (c[e] & 0x80) ? 0.0 : v[e];
+          double o = (c[e] & 0x80) ? 0.0 : (double)v[e];
           if (hd[e].y != 0) o = (hd[e].y == 2 && t == 0) ? G.diag : 0.0;
           data[(int64_t)hd[e].x + t] = (OT)o;
         }
@@ -617,9 +631,9 @@ __global__ void __launch_bounds__(BS) k_fused(const hm_problem P, int s, int j, 
             const int lz = pt * ez + a3;
             if (lz < lz_lo || lz >= lz_hi) continue;  // a row of the chunk below (halo)
             const int wz = wz_of(lz);
-            double* row = ((lz & 1) ? BUF1 : BUF0) + (size_t)b2d * wz;
+            BT* row = ((lz & 1) ? BUF1 : BUF0) + (size_t)b2d * wz;
             const int oz0 = pc * ((a3 == 0 && ez > 0) ? 1 : 0);
-            double old[NC][NC];
+            BT old[NC][NC];
 #pragma unroll
             for (int i = 0; i < NC; ++i)
 #pragma unroll
@@ -630,7 +644,7 @@ __global__ void __launch_bounds__(BS) k_fused(const hm_problem P, int s, int j, 
               for (int b3 = 0; b3 < NC; ++b3) {
                 const CT v = acc[i][a3][b3];
                 bad |= !__builtin_isfinite(v);
-                row[ox + wx * ((oy0 + i) + wy * (oz0 + b3))] = old[i][b3] + (double)v;
+                row[ox + wx * ((oy0 + i) + wy * (oz0 + b3))] = old[i][b3] + (BT)v;
               }
           }
         }
@@ -658,6 +672,17 @@ __global__ void __launch_bounds__(BS) k_fused(const hm_problem P, int s, int j, 
   if (bad && P.nonfinite) atomicOr(P.nonfinite, 1);
 }
 
+// Whether a tile of sizes z (bx x by elements, test 1-D order nt - 1) runs in blocks of bs threads:
+// a task a thread, eight lanes an element slot (the alignment test), a row a thread (the plane
+// tables), and the element data and stencil maps of a layer in FU_ITEMS, FU_SIGV loads a thread.
+static bool fu_fits(const hm_problem& P, const SumfactPlan& SP, const FusedSizes& z, int bs, int bx, int by, int nt) {
+  const int items = (z.m_max * (8 + P.nd_m + (SP.need_fields ? P.nd_u : 0)) + bs - 1) / bs;
+  const int sigv = (int)((FU_SLOTS * z.sig_bytes / 16 + bs - 1) / bs);
+  return 8 * z.m_max <= bs && z.t_max <= bs && z.t_max <= FU_MAXT && z.m_max <= FU_MAXM && items <= FU_ITEMS &&
+         sigv <= FU_SIGV && z.rows_max <= bs && z.rows_max <= 256 && (size_t)z.sx_max * z.sy_max <= FU_MAXS2D &&
+         z.sig_bytes <= 8192 && (nt - 1) * bx + 1 <= FU_MAXL && (nt - 1) * by + 1 <= FU_MAXL;
+}
+
 template <typename CT, int NT, int NC, int NQ>
 static FusedSizes fu_pick(const hm_problem& P, const SumfactPlan& SP, int nslot, bool aligned, int bs, size_t limit,
                           int* bx, int* by) {
@@ -665,11 +690,7 @@ static FusedSizes fu_pick(const hm_problem& P, const SumfactPlan& SP, int nslot,
   FusedSizes z{};
   for (const auto& c : cand) {
     z = fu_sizes<CT, NT, NC, NQ>(c[0], c[1], nslot, P.nd_u, P.nd_m, SP.need_fields != 0, aligned, FU_MAXZ);
-    const int items = (z.m_max * (8 + P.nd_m + (SP.need_fields ? P.nd_u : 0)) + bs - 1) / bs;
-    const int sigv = (int)((FU_SLOTS * z.sig_bytes / 16 + bs - 1) / bs);
-    if (z.bytes <= limit && z.t_max <= bs && z.t_max <= FU_MAXT && z.m_max <= FU_MAXM && items <= FU_ITEMS &&
-        sigv <= FU_SIGV && z.rows_max <= bs && z.rows_max <= 256 && (size_t)z.sx_max * z.sy_max <= FU_MAXS2D &&
-        z.sig_bytes <= 8192 && (NT - 1) * c[0] + 1 <= FU_MAXL && (NT - 1) * c[1] + 1 <= FU_MAXL) {
+    if (z.bytes <= limit && fu_fits(P, SP, z, bs, c[0], c[1], NT)) {
       *bx = c[0];
       *by = c[1];
       return z;
@@ -683,12 +704,18 @@ static FusedSizes fu_pick(const hm_problem& P, const SumfactPlan& SP, int nslot,
 constexpr size_t FU_STATIC_LDS =
     sizeof(int32_t) * (32 + FU_MAXT + FU_MAXL * FU_MAXL + 2 * FU_MAXL + 2 * FU_MAXM + 1) + FU_MAXS2D;
 
-static size_t lds_limit() {
-  int dev = 0, v = 0;
+// The dynamic LDS a block may take when `blocks` blocks share a compute unit.
+static size_t lds_limit(int blocks = 1) {
+  int dev = 0, v = 0, w = 0;
   size_t lim = 64 * 1024;
   if (hipGetDevice(&dev) == hipSuccess &&
       hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
     lim = (size_t)v;
+  if (blocks > 1) {
+    if (hipDeviceGetAttribute(&w, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess || w <= 0)
+      return 0;
+    lim = std::min(lim, ((size_t)w / blocks) & ~(size_t)1023);
+  }
   return lim > FU_STATIC_LDS ? lim - FU_STATIC_LDS : 0;
 }
 
@@ -743,28 +770,57 @@ static int fu_plan(const hm_problem* P, int i, int j, SumfactPlan& SP, int* nt, 
   return HM_OK;
 }
 
-// The tile of block (i, j): the largest candidate that fits the general form (every element
-// tested for its alignment), so that a plan made for it serves a mesh of either kind.
+// The blocks of (i, j) that share a compute unit on tile bx x by, in the form of the mesh's kind:
+// one of 768 threads (512 for the 4-point rule); the FP32 form of the 3-point rule two of 512
+// where the tile's LDS fits half of the unit's (their barriers and load waits overlap).
 template <typename CT, int NT, int NC, int NQ>
-static int fu_tile(const hm_problem* P, const SumfactPlan& SP, int i, int j, int* bx, int* by) {
+static int fu_blocks(const hm_problem& P, const SumfactPlan& SP, int i, int j, bool aligned, int bx, int by) {
+  if (!(std::is_same<CT, float>::value && NQ == 3)) return 1;
+  const FusedSizes Z = fu_sizes<CT, NT, NC, NQ>(bx, by, fu_slots(SP, i, j, aligned).nslot, P.nd_u, P.nd_m,
+                                                SP.need_fields != 0, aligned, FU_MAXZ);
+  return Z.bytes <= lds_limit(2) && fu_fits(P, SP, Z, 512, bx, by, NT) ? 2 : 1;
+}
+
+// The tile of block (i, j): the largest candidate that fits the general form (every element
+// tested for its alignment), so that a plan made for it serves a mesh of either kind; and the
+// blocks a compute unit on it in the form of flags (nb).
+template <typename CT, int NT, int NC, int NQ>
+static int fu_tile(const hm_problem* P, const SumfactPlan& SP, int i, int j, int flags, int* bx, int* by, int* nb) {
   constexpr int BS = NQ == 3 ? 768 : 512;
   fu_pick<CT, NT, NC, NQ>(*P, SP, fu_slots(SP, i, j, false).nslot, false, BS, lds_limit(), bx, by);
   if (*bx == 0) HM_FAIL(HM_ERR_UNSUPPORTED, "no tile of the fused assembly fits the LDS");
+  *nb = fu_blocks<CT, NT, NC, NQ>(*P, SP, i, j, (flags & HM_FUSED_ALIGNED) != 0, *bx, *by);
   return HM_OK;
 }
 
-template <typename CT, int NT, int NC, int QP, int NQ, typename OT, bool AL>
-static int fu_launch(const hm_problem& P, int i, int j, const SumfactPlan& SP, const hm_fused_plan& F, void* data,
-                     hipStream_t st) {
-  constexpr int BS = NQ == 3 ? 768 : 512;
-  const FuSlots W = fu_slots(SP, i, j, AL);
-  int bx = 0, by = 0;
-  fu_pick<CT, NT, NC, NQ>(P, SP, fu_slots(SP, i, j, false).nslot, false, BS, lds_limit(), &bx, &by);
-  if (bx != F.bx || by != F.by)
-    HM_FAIL(HM_ERR_ARG, "the plan is laid out for tiles of %d x %d, this block takes %d x %d", F.bx, F.by, bx, by);
-  const FusedSizes Z =
-      fu_sizes<CT, NT, NC, NQ>(bx, by, W.nslot, P.nd_u, P.nd_m, SP.need_fields != 0, AL, FU_MAXZ);
-  auto kern = k_fused<CT, NT, NC, QP, NQ, OT, BS, AL>;
+template <typename CT>
+static int fu_tile_orders(const hm_problem* P, const SumfactPlan& SP, int i, int j, int nt, int nc, int flags,
+                          int* bx, int* by, int* nb) {
+  if (P->nq1 == 4) {
+    if (nt == 3 && nc == 3) return fu_tile<CT, 3, 3, 4>(P, SP, i, j, flags, bx, by, nb);
+    if (nt == 3) return fu_tile<CT, 3, 2, 4>(P, SP, i, j, flags, bx, by, nb);
+    return fu_tile<CT, 2, 2, 4>(P, SP, i, j, flags, bx, by, nb);
+  }
+  if (nt == 3 && nc == 3) return fu_tile<CT, 3, 3, 3>(P, SP, i, j, flags, bx, by, nb);
+  if (nt == 3) return fu_tile<CT, 3, 2, 3>(P, SP, i, j, flags, bx, by, nb);
+  return fu_tile<CT, 2, 2, 3>(P, SP, i, j, flags, bx, by, nb);
+}
+
+static int fu_query(const hm_problem* P, int i, int j, int precision, int flags, int* bx, int* by, int* nb) {
+  if (precision != HM_FP64 && precision != HM_FP32)
+    HM_FAIL(HM_ERR_UNSUPPORTED, "fused assembly in FP64 or FP32 only");
+  SumfactPlan SP;
+  int nt = 0, nc = 0;
+  if (int rc = fu_plan(P, i, j, SP, &nt, &nc)) return rc;
+  if (precision == HM_FP64) return fu_tile_orders<double>(P, SP, i, j, nt, nc, flags, bx, by, nb);
+  return fu_tile_orders<float>(P, SP, i, j, nt, nc, flags, bx, by, nb);
+}
+
+template <typename CT, int NT, int NC, int QP, int NQ, typename OT, int BS, int NB, bool AL>
+static int fu_run(const hm_problem& P, int i, int j, const SumfactPlan& SP, const FuSlots& W, const FusedSizes& Z,
+                  const hm_fused_plan& F, void* data, hipStream_t st) {
+  const int bx = F.bx, by = F.by;
+  auto kern = k_fused<CT, NT, NC, QP, NQ, OT, BS, NB, AL>;
   if (Z.bytes > 48 * 1024) {
     const hipError_t err = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                                (int)Z.bytes);
@@ -800,13 +856,14 @@ static int fu_launch(const hm_problem& P, int i, int j, const SumfactPlan& SP, c
       }
       G.pn[kind][t] = n;
     }
-  // layers a chunk: the fewest rounds of blocks over the compute units times the layers a
-  // block sweeps (one more for the halo layer of a cut column)
+  // layers a chunk: the fewest rounds of blocks over the compute units (NB blocks on each)
+  // times the layers a block sweeps (one more for the halo layer of a cut column)
   int cus = 256, dev = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     cus = 256;
   if (cus < 1) cus = 256;
+  cus *= NB;
   const int64_t ntile = (int64_t)G.ntx * G.nty;
   int best = 1;
   int64_t best_cost = INT64_MAX;
@@ -829,6 +886,24 @@ static int fu_launch(const hm_problem& P, int i, int j, const SumfactPlan& SP, c
                      (const double*)(i == HM_PARAMETER ? P.tab1_m : P.tab1_u),
                      (const double*)(j == HM_PARAMETER ? P.tab1_m : P.tab1_u), (OT*)data);
   return hip_check("k_fused");
+}
+
+template <typename CT, int NT, int NC, int QP, int NQ, typename OT, bool AL>
+static int fu_launch(const hm_problem& P, int i, int j, const SumfactPlan& SP, const hm_fused_plan& F, void* data,
+                     hipStream_t st) {
+  constexpr int BS = NQ == 3 ? 768 : 512;
+  const FuSlots W = fu_slots(SP, i, j, AL);
+  int bx = 0, by = 0;
+  fu_pick<CT, NT, NC, NQ>(P, SP, fu_slots(SP, i, j, false).nslot, false, BS, lds_limit(), &bx, &by);
+  if (bx != F.bx || by != F.by)
+    HM_FAIL(HM_ERR_ARG, "the plan is laid out for tiles of %d x %d, this block takes %d x %d", F.bx, F.by, bx, by);
+  const FusedSizes Z =
+      fu_sizes<CT, NT, NC, NQ>(bx, by, W.nslot, P.nd_u, P.nd_m, SP.need_fields != 0, AL, FU_MAXZ);
+  if constexpr (std::is_same<CT, float>::value && NQ == 3) {
+    if (fu_blocks<CT, NT, NC, NQ>(P, SP, i, j, AL, bx, by) == 2)
+      return fu_run<CT, NT, NC, QP, NQ, OT, 512, 2, AL>(P, i, j, SP, W, Z, F, data, st);
+  }
+  return fu_run<CT, NT, NC, QP, NQ, OT, BS, 1, AL>(P, i, j, SP, W, Z, F, data, st);
 }
 
 template <typename CT, int QP, typename OT, bool AL>
@@ -858,24 +933,18 @@ using namespace hm;
 
 extern "C" int hm_fused_tile(const hm_problem* P, int i, int j, int precision, int32_t* bx, int32_t* by) {
   HM_REQUIRE(P && bx && by, "bad fused tile arguments");
-  if (precision != HM_FP64 && precision != HM_FP32)
-    HM_FAIL(HM_ERR_UNSUPPORTED, "fused assembly in FP64 or FP32 only");
-  SumfactPlan SP;
-  int nt = 0, nc = 0;
-  if (int rc = fu_plan(P, i, j, SP, &nt, &nc)) return rc;
-  int x = 0, y = 0, rc = HM_OK;
-  const bool f64 = precision == HM_FP64;
-  if (P->nq1 == 4) {
-    rc = nt == 3 && nc == 3 ? (f64 ? fu_tile<double, 3, 3, 4>(P, SP, i, j, &x, &y) : fu_tile<float, 3, 3, 4>(P, SP, i, j, &x, &y))
-         : nt == 3          ? (f64 ? fu_tile<double, 3, 2, 4>(P, SP, i, j, &x, &y) : fu_tile<float, 3, 2, 4>(P, SP, i, j, &x, &y))
-                            : (f64 ? fu_tile<double, 2, 2, 4>(P, SP, i, j, &x, &y) : fu_tile<float, 2, 2, 4>(P, SP, i, j, &x, &y));
-  } else {
-    rc = nt == 3 && nc == 3 ? (f64 ? fu_tile<double, 3, 3, 3>(P, SP, i, j, &x, &y) : fu_tile<float, 3, 3, 3>(P, SP, i, j, &x, &y))
-         : nt == 3          ? (f64 ? fu_tile<double, 3, 2, 3>(P, SP, i, j, &x, &y) : fu_tile<float, 3, 2, 3>(P, SP, i, j, &x, &y))
-                            : (f64 ? fu_tile<double, 2, 2, 3>(P, SP, i, j, &x, &y) : fu_tile<float, 2, 2, 3>(P, SP, i, j, &x, &y));
-  }
+  int x = 0, y = 0, nb = 1;
+  const int rc = fu_query(P, i, j, precision, 0, &x, &y, &nb);
   *bx = x;
   *by = y;
+  return rc;
+}
+
+extern "C" int hm_fused_blocks(const hm_problem* P, int i, int j, int precision, int32_t flags, int32_t* blocks) {
+  HM_REQUIRE(P && blocks, "bad fused blocks arguments");
+  int x = 0, y = 0, nb = 1;
+  const int rc = fu_query(P, i, j, precision, flags, &x, &y, &nb);
+  *blocks = nb;
   return rc;
 }
 

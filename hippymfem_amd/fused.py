@@ -64,6 +64,21 @@ def tile_of(kernel, i, j, precision):
     return int(bx.value), int(by.value)
 
 
+def blocks_of(kernel, i, j, precision, aligned):
+    """Blocks of the fused kernel that share a compute unit for this block on a mesh of the
+    kind ``aligned`` (``hm_fused_blocks``): 2 where the FP32 form's tile fits half the unit's
+    LDS, else 1."""
+    P, _ = kernel.problem([_zeros(kernel, s) for s in range(3)])
+    nb = _lib.I32(0)
+    rc = _lib.lib().hm_fused_blocks(_lib.ctypes.byref(P), int(i), int(j),
+                                    _lib.PRECISIONS[precision],
+                                    _lib.FUSED_ALIGNED if aligned else 0, _lib.ctypes.byref(nb))
+    if rc == _lib.HM_ERR_UNSUPPORTED:
+        return 1
+    _lib.check(rc, "fused blocks")
+    return int(nb.value)
+
+
 def _zeros(kernel, s):
     return np.zeros(1)
 

@@ -184,6 +184,17 @@ class BlockAssembly:
             return False
         return want <= 0.5 * torch.cuda.mem_get_info(self.data.device)[0]
 
+    def _fp32_two_blocks(self):
+        """Whether the fused kernel of this block runs two blocks a compute unit (its FP32 form
+        on a mesh of axis-aligned boxes, where the tile's LDS fits half the unit's)."""
+        if self.precision != "fp32" or fused.supported(self.kernel, self.i, self.j,
+                                                       self.precision) is not None:
+            return False
+        mesh = self.kernel.batches.mesh
+        if not fused.aligned_box(mesh):
+            return False
+        return fused.blocks_of(self.kernel, self.i, self.j, self.precision, True) == 2
+
     @property
     def dplan(self):
         """The scatter's device plan (codes in run order, row modes), made at the first
@@ -213,11 +224,14 @@ class BlockAssembly:
                 why = "values of another precision than the element matrices"
             elif self.plan.diag_policy == "keep" and self.plan.diag is not None:
                 why = "diag_policy 'keep'"
-            elif self.route == "auto" and self.precision != "fp64" and self._e_fits():
-                # single precision: FP32 element matrices halve the two-pass route's traffic,
-                # while the fused kernel's time is its FP64 row sums and point work (64^3 P2 A:
-                # 0.93 ms two-pass against 1.08 ms fused in FP32)
-                why = "single precision (the two-pass route is faster)"
+            elif self.route == "auto" and self.precision != "fp64" and self._e_fits() and \
+                    not self._fp32_two_blocks():
+                # single precision: FP32 element matrices halve the two-pass route's traffic;
+                # the fused kernel gains on it only where two of its blocks share a compute
+                # unit, on box meshes (64^3 P2 A: 0.68 ms fused against 0.92 ms two-pass; with
+                # one block, perturbed: 1.26 ms fused against 1.15 ms two-pass)
+                why = "single precision with one fused block a compute unit (the two-pass " \
+                      "route is faster)"
             elif self.route == "auto" and self.kernel.group.nq1 > 3:
                 # the 4-point rule: 64 points' pairs in LDS leave the fused kernel small tiles
                 # (64^3 P2 A, degree 6: 2.25 ms fused against 1.71 ms two-pass)

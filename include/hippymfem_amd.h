@@ -267,7 +267,9 @@ int hm_csr_matvec(int64_t nrow, const int32_t* indptr, const int32_t* indices,
  * each block owning the CSR rows of its lattice points and recomputing a one-element halo on its
  * lower sides; every CSR value is the sum of its entries in ascending element order, the order of
  * np.bincount (hippymfem/fem/pattern.py:788), with the element-matrix kernel's bits for every
- * entry: the values are those of hm_element_matrices followed by hm_scatter_matrix, bit for bit.
+ * entry.  FP64: the values are those of hm_element_matrices followed by hm_scatter_matrix, bit for
+ * bit.  FP32: the FP32 entries are summed in FP32 in that order (hm_scatter_matrix sums them in
+ * FP64), so that the sums take half the LDS and two blocks may share a compute unit.
  * The plan holds, in tile order (planes of the test lattice, then tiles x fastest, then the
  * tile's lattice points x fastest), every row's CSR offset and row mode (0 assemble, 1 zero row,
  * 2 zero row with diag first) and every CSR entry's place in the row's dense stencil (the box of
@@ -301,6 +303,12 @@ typedef struct hm_fused_plan {
 #define HM_FUSED_ALIGNED 1
 
 int hm_fused_tile(const hm_problem* P, int i, int j, int precision, int32_t* bx, int32_t* by);
+
+/* hm_fused_blocks: no counterpart in the reference (the route choice of the fused assembly).
+ * The blocks of the fused kernel that share a compute unit for block (i, j) at this precision on
+ * a plan with these flags: 1, or 2 where the FP32 form's tile fits half the unit's LDS (its plane
+ * sums then FP32).  Errors as hm_fused_tile. */
+int hm_fused_blocks(const hm_problem* P, int i, int j, int precision, int32_t flags, int32_t* blocks);
 int hm_assemble_fused(const hm_problem* P, int i, int j, const hm_fused_plan* F, int precision,
                       void* data, int out_dtype, void* stream);
 

@@ -1,9 +1,10 @@
 """The fused route (element kernel and scatter in one kernel, ``hm_assemble_fused``) against
 the two-pass route and the oracle.
 
-The fused values are the two-pass values bit for bit (every entry has the element kernel's
-bits and every CSR value sums its entries in ascending element order, as ``np.bincount``
-does), for every block, numbering, mesh, rule and tile edge; and within 1e-13 of the oracle.
+The FP64 fused values are the two-pass values bit for bit (every entry has the element
+kernel's bits and every CSR value sums its entries in ascending element order, as
+``np.bincount`` does), for every block, numbering, mesh, rule and tile edge; and within 1e-13
+of the oracle.  The FP32 form sums the same FP32 entries in FP32 in the same order.
 """
 
 import numpy as np
@@ -67,13 +68,61 @@ def test_fused_equals_two_pass_and_oracle(gpu, n, numbering, nonlinear, perturb,
         assert rel(d, do) <= TOL, (ij, rel(d, do))
 
 
-@pytest.mark.parametrize("nonlinear", [False, True])
-def test_fused_fp32_equals_two_pass(gpu, nonlinear):
-    W = Workload(6, nonlinear=nonlinear, perturb=0.2)
-    for ij in BLOCKS:
-        A, B, _, _ = both_routes(W, ij, precision="fp32", nonlinear=nonlinear)
-        assert A.data.dtype == torch.float32
-        assert torch.equal(A.data, B.data), ij
+def fp32_sums_in_element_order(t):
+    """The FP32 form's CSR values restated on the host: each value the FP32 sum, from 0, of its
+    entries in ascending element order, the entries those of the element kernel (the two-pass
+    block ``t``'s FP32 element matrices); the slots of the elimination as ``t`` writes them."""
+    pat = t.pattern
+    E = t.E.cpu().numpy()
+    ne, nt, nc = E.shape
+    rows = np.repeat(pat.tdofs[:, :, None], nc, axis=2).reshape(-1).astype(np.int64)
+    cols = np.repeat(pat.cdofs[:, None, :], nt, axis=1).reshape(-1).astype(np.int64)
+    indptr = pat.indptr.astype(np.int64)
+    row_of = np.repeat(np.arange(pat.nrow, dtype=np.int64), np.diff(indptr))
+    key = row_of * pat.ncol + pat.indices.astype(np.int64)
+    order = np.argsort(key)
+    want = rows * pat.ncol + cols
+    pos = order[np.searchsorted(key[order], want)]
+    assert np.array_equal(key[pos], want)
+    srt = np.argsort(pos, kind="stable")            # entries are element-major: stable keeps e order
+    p, v = pos[srt], E.reshape(-1)[srt]
+    start = np.r_[0, np.flatnonzero(np.diff(p)) + 1]
+    rank = np.arange(p.size) - np.repeat(start, np.diff(np.r_[start, p.size]))
+    out = np.zeros(pat.nnz, dtype=np.float32)
+    for r in range(int(rank.max()) + 1):
+        sel = rank == r
+        out[p[sel]] = out[p[sel]] + v[sel]
+    kill = t.plan.kill
+    if kill is not None and len(kill):
+        out[kill] = t.data.cpu().numpy()[kill]
+    return out
+
+
+@pytest.mark.parametrize("n,nonlinear,perturb", [(6, False, 0.2), (6, True, 0.2), (9, False, 0.0),
+                                                 (5, True, 0.0)])
+def test_fused_fp32_sums_in_fp32_in_element_order(gpu, n, nonlinear, perturb):
+    """The FP32 form sums its FP32 entries in FP32, in ascending element order: its values are
+    the host's FP32 sums of the element kernel's entries in that order, bit for bit, on
+    perturbed meshes (one block a compute unit) and box meshes (two, the 9^3 case in chunks of
+    layers); and within 1e-6 of the FP64 values."""
+    W = Workload(n, nonlinear=nonlinear, perturb=perturb)
+    prob = PDEVariationalProblem(W.spaces, W.density, W.batches, W.bc, is_fwd_linear=not nonlinear)
+    loc = [torch.as_tensor(v).cuda() for v in (W.u, W.m, W.p)]
+    for ij in BLOCKS + ([(STATE, STATE)] if nonlinear else []):
+        print("case n=%d nl=%s perturb=%g block %s" % (n, nonlinear, perturb, ij), flush=True)
+        f = prob.block_assembly(ij[0], ij[1], "fp32", route="fused")
+        t = prob.block_assembly(ij[0], ij[1], "fp32", route="two-pass")
+        r = prob.block_assembly(ij[0], ij[1], "fp64", route="two-pass")
+        A = f.assemble(loc).data.clone()
+        t.assemble(loc)
+        assert f.fused_plan() is not None and t.fused_plan() is None
+        assert A.dtype == torch.float32
+        want = fp32_sums_in_element_order(t)
+        got = A.cpu().numpy()
+        assert np.array_equal(got, want), (ij, int((got != want).sum()))
+        ref = r.assemble(loc).data
+        assert rel(got, ref.cpu().numpy()) <= 1e-6, ij
+        assert torch.equal(f.assemble(loc).data, A), ij
 
 
 def test_fused_repeat_bit_identical_and_linearization_point(gpu):
