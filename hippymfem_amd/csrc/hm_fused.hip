@@ -124,6 +124,10 @@ __host__ inline FusedSizes fu_sizes(int bx, int by, int nslot, int nd_u, int nd_
   return z;
 }
 
+// The sub-step of the adds of a task's rows (local x a1, y a2 within its element): rows of local
+// y >= 1 before those of y = 0, and within each, local x >= 1 before x = 0.
+__host__ __device__ inline int fu_group(int a1, int a2) { return (a2 >= 1 ? 0 : 2) + (a1 >= 1 ? 0 : 1); }
+
 // BS threads a block, NB blocks a compute unit (the register budget: NB BS / 256 waves a SIMD).
 template <typename CT, int NT, int NC, int QP, int NQ, typename OT, int BS, int NB, bool AL>
 __global__ void __attribute__((amdgpu_flat_work_group_size(1, BS), amdgpu_waves_per_eu((NB * BS + 255) / 256)))
@@ -138,7 +142,7 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
   __shared__ int32_t B2D[FU_MAXL * FU_MAXL];
   __shared__ int32_t WXs[FU_MAXL], WYs[FU_MAXL];
   __shared__ int32_t EFLAG[FU_MAXM];
-  __shared__ int32_t TCOUNT[FU_MAXM + 1];
+  __shared__ int32_t TCOUNT[4];            // tasks of each sub-step group
   __shared__ uint8_t ROWOF[FU_MAXS2D];  // the row of every stencil place of a unit-height plane
 
   const int nu = P.nd_u, nm = P.nd_m;
@@ -205,12 +209,18 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
     // stencil widths of the tile's lattice points, and every row's place in a plane buffer
     for (int i = tid; i < cxl; i += BS) WXs[i] = fu_width(lx_lo + i, pt, G.nx, pc);
     for (int i = tid; i < cyl; i += BS) WYs[i] = fu_width(ly_lo + i, pt, G.ny, pc);
-    // contraction tasks of each element slot: (a1, a2) of its owned rows, every b1
-    for (int m = tid; m < M; m += BS) {
-      const int ex = xs + m % cx, ey = ys + m / cx;
-      const int a1n = min(lx_hi - pt * ex, NT) - max(lx_lo - pt * ex, 0);
-      const int a2n = min(ly_hi - pt * ey, NT) - max(ly_lo - pt * ey, 0);
-      TCOUNT[m + 1] = a1n * a2n * NC;
+    // contraction tasks: (a1, a2) of the owned rows of every element slot, every b1, counted by
+    // the sub-step of their adds (thread g counts group g)
+    if (tid < 4) {
+      int n = 0;
+      for (int m = 0; m < M; ++m) {
+        const int ex = xs + m % cx, ey = ys + m / cx;
+        const int a1lo = max(lx_lo - pt * ex, 0), a1hi = min(lx_hi - pt * ex, NT);
+        const int a2lo = max(ly_lo - pt * ey, 0), a2hi = min(ly_hi - pt * ey, NT);
+        for (int a2 = a2lo; a2 < a2hi; ++a2)
+          for (int a1 = a1lo; a1 < a1hi; ++a1) n += fu_group(a1, a2) == tid ? NC : 0;
+      }
+      TCOUNT[tid] = n;
     }
   }
   const int64_t hdr0 = G.hdr_off[tile];  // the tile's first row in every plane (plane 0's)
@@ -226,34 +236,49 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
       for (int ix = 0; ix < cxl; ++ix) B2D[iy * cxl + ix] = sx * py + WYs[iy] * px[ix];
       py += WYs[iy];
     }
-    TCOUNT[0] = 0;
-    for (int m = 0; m < M; ++m) TCOUNT[m + 1] += TCOUNT[m];
   }
   __syncthreads();
-  for (int m = tid; m < M; m += BS) {
-    const int ex = xs + m % cx, ey = ys + m / cx;
-    const int a1lo = max(lx_lo - pt * ex, 0), a1hi = min(lx_hi - pt * ex, NT);
-    const int a2lo = max(ly_lo - pt * ey, 0), a2hi = min(ly_hi - pt * ey, NT);
-    int k = TCOUNT[m];
-    for (int a2 = a2lo; a2 < a2hi; ++a2)
-      for (int a1 = a1lo; a1 < a1hi; ++a1)
-        for (int b1 = 0; b1 < NC; ++b1) TASK[k++] = m | (a1 << 8) | (b1 << 12) | (a2 << 16);
+  // the tasks in the order of the sub-steps (thread g writes group g), in each group by element
+  // slot, a2, a1, b1: the lanes of a wavefront mostly share one sub-step, and a wavefront passes
+  // over the others as a whole
+  if (tid < 4) {
+    int k = 0;
+    for (int g = 0; g < tid; ++g) k += TCOUNT[g];
+    for (int m = 0; m < M; ++m) {
+      const int ex = xs + m % cx, ey = ys + m / cx;
+      const int a1lo = max(lx_lo - pt * ex, 0), a1hi = min(lx_hi - pt * ex, NT);
+      const int a2lo = max(ly_lo - pt * ey, 0), a2hi = min(ly_hi - pt * ey, NT);
+      for (int a2 = a2lo; a2 < a2hi; ++a2)
+        for (int a1 = a1lo; a1 < a1hi; ++a1)
+          if (fu_group(a1, a2) == tid)
+            for (int b1 = 0; b1 < NC; ++b1) TASK[k++] = m | (a1 << 8) | (b1 << 12) | (a2 << 16);
+    }
   }
   for (int rho = tid; rho < nrows; rho += BS) {
     const int b = B2D[rho], w = WXs[rho % cxl] * WYs[rho / cxl];
     for (int q = 0; q < w; ++q) ROWOF[b + q] = (uint8_t)rho;
   }
   __syncthreads();
-  const int ntask = TCOUNT[M];
+  const int ntask = TCOUNT[0] + TCOUNT[1] + TCOUNT[2] + TCOUNT[3];
   const int s2d_row = B2D[nrows - 1] + WXs[cxl - 1] * WYs[cyl - 1];  // stencil entries / wz
 
   // ------------------------------------------------------------------ element data
   // items of a layer: per element slot 8 vertices, nd_m parameter dofs, and (fields) nd_u
   // state/adjoint dofs; thread tid takes items tid, tid + BS, ...  A layer's data is loaded
   // into registers during the layer before (after its contraction, so that the loads are in
-  // flight during its adds and are not queued behind its stores) and put in LDS at the top of
-  // its own layer.
+  // flight during its first adds and are not queued behind its stores) and put in LDS after
+  // those adds, before the layer's planes are written out (a wait for the loads there waits for
+  // no store).  The arrays are read through pointers held in scalar registers: a per-lane choice
+  // of the problem's fields would read the pointers themselves from memory, one wait on every
+  // store before it.
   const int NI = 8 + nm + (fields ? nu : 0);
+  const int32_t* const ev_ = P.elem_verts;
+  const int32_t* const em_ = P.edofs_m;
+  const int32_t* const eu_ = P.edofs_u;
+  const double* const xv_ = P.verts;
+  const double* const mv_ = P.m;
+  const double* const uv_ = P.u;
+  const double* const pv_ = P.p;
   int32_t nid[FU_ITEMS];
   double nv[FU_ITEMS][3];
   auto item_elem = [&](int it, int ez) -> int64_t {
@@ -268,12 +293,10 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
       if (it >= M * NI || ez >= z1) continue;
       const int64_t e = item_elem(it, ez);
       const int w = it % NI;
-      if (w < 8)
-        nid[k] = P.elem_verts[e * 8 + w];
-      else if (w < 8 + nm)
-        nid[k] = P.edofs_m[e * nm + (w - 8)];
-      else
-        nid[k] = P.edofs_u[e * nu + (w - 8 - nm)];
+      // one load an item, its source chosen per lane (no branch: loads of divergent branches
+      // into one register would wait for each other)
+      const int32_t* src = w < 8 ? ev_ + (e * 8 + w) : w < 8 + nm ? em_ + (e * nm + (w - 8)) : eu_ + (e * nu + (w - 8 - nm));
+      nid[k] = *src;
     }
   };
   auto load_values = [&](int ez) {
@@ -283,16 +306,15 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
       if (it >= M * NI || ez >= z1) continue;
       const int w = it % NI;
       const int64_t d = nid[k];
-      if (w < 8) {
-        nv[k][0] = P.verts[d * 3];
-        nv[k][1] = P.verts[d * 3 + 1];
-        nv[k][2] = P.verts[d * 3 + 2];
-      } else if (w < 8 + nm) {
-        nv[k][0] = P.m[d];
-      } else {
-        nv[k][0] = P.u[d];
-        nv[k][1] = P.p[d];
-      }
+      // three loads an item, whatever its kind: a vertex's coordinates, a parameter dof's value
+      // (three times), a state dof's u and p
+      const bool vx = w < 8, vm = !vx && w < 8 + nm;
+      const double* s0 = vx ? xv_ + d * 3 : vm ? mv_ + d : uv_ + d;
+      const double* s1 = vx ? s0 + 1 : vm ? s0 : pv_ + d;
+      const double* s2 = vx ? s0 + 2 : s0;
+      nv[k][0] = *s0;
+      nv[k][1] = *s1;
+      nv[k][2] = *s2;
     }
   };
   auto store_values = [&](int ez) {
@@ -381,7 +403,7 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
   // plane's places tid, tid + BS, ... (in row order, so the stores of a row are contiguous),
   // every read of the batch issued before its first use.  A place's row: ROWOF[k / wz] (a row
   // of the plane holds w_x w_y wz places from B2D wz on); k / wz by a multiply and a shift
-  // (exact for k < 2^16 / wz).
+  // (exact for k < 2^16 / wz).  The index products are of 24-bit operands (full-rate multiplies).
   auto write_plane = [&](int lz, int k) {
     rederive();
     const int wz = wz_of(lz);
@@ -401,13 +423,13 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
       for (int e = 0; e < FU_EB; ++e) {
         kk[e] = base + tid + e * BS;
         const int kc = min(kk[e], S - 1);
-        rho[e] = ROWOF[((uint32_t)kc * mul) >> 16];
+        rho[e] = ROWOF[__umul24((uint32_t)kc, mul) >> 16];
         c[e] = sg[kc];
       }
 #pragma unroll
       for (int e = 0; e < FU_EB; ++e) {
         hd[e] = hdr[rho[e]];
-        rb[e] = B2D[rho[e]] * wz;
+        rb[e] = (int)__umul24((uint32_t)B2D[rho[e]], (uint32_t)wz);
       }
 #pragma unroll
       for (int e = 0; e < FU_EB; ++e) v[e] = buf[rb[e] + (c[e] & 0x7F)];
@@ -428,12 +450,12 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
   CT acc[NC][NT][NC];
   load_ids(zs);
   load_values(zs);
+  store_values(zs);
   load_ids(zs + 1);
   for (int ez = zs; ez < z1; ++ez) {
     rederive();
-    // 1. this layer's element data, loaded during the layer before; the tables of the planes
-    // it completes in flight during the point phase
-    store_values(ez);
+    // 1. this layer's element data, put in LDS by the layer before; the tables of the planes it
+    // completes in flight during the point phase
     __syncthreads();
     load_planes(ez);
     // each element's alignment (every vertex coordinate that of the vertex with the same index
@@ -606,12 +628,13 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
     // lane are distinct, so a lane reads all of them before it writes any.
     const int ex = xs + m % cx, ey = ys + m / cx;
     const int ix = pt * ex + a1 - lx_lo, iy = pt * ey + a2 - ly_lo;
-    const int grp = (a2 >= 1 ? 0 : 2) + (a1 >= 1 ? 0 : 1);
+    const int grp = fu_group(a1, a2);
     const int ox = pc * ((a1 == 0 && ex > 0) ? 1 : 0) + b1;
     const int oy0 = pc * ((a2 == 0 && ey > 0) ? 1 : 0);
     const int wx = has ? WXs[ix] : 0, wy = has ? WYs[iy] : 0;
     const int b2d = has ? B2D[iy * cxl + ix] : 0;
-    int rk = 0;  // staged planes written so far
+    int rk = 0;            // staged planes written so far
+    bool staged = false;   // the next layer's element data in LDS
     constexpr int NA = NT < 2 ? NT : 2;  // rows of phase A: a3 < NA
 #pragma unroll
     for (int ph = 0; ph < 2; ++ph) {
@@ -650,6 +673,12 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
         }
         __syncthreads();
       }
+      // the next layer's element data into LDS (its loads had the adds to arrive; this layer's
+      // point phase, the only reader of the region, is over), before this phase's stores
+      if (!staged) {
+        store_values(ez + 1);
+        staged = true;
+      }
       // the planes of this phase no later layer adds to
       for (int a3 = a3lo; a3 < a3hi; ++a3) {
         const int lz = pt * ez + a3;
@@ -668,6 +697,7 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
         }
       }
     }
+    if (!staged) store_values(ez + 1);
   }
   if (bad && P.nonfinite) atomicOr(P.nonfinite, 1);
 }
@@ -702,7 +732,7 @@ static FusedSizes fu_pick(const hm_problem& P, const SumfactPlan& SP, int nslot,
 
 // the static LDS of k_fused (task table, row places, widths, flags), left out of its dynamic LDS
 constexpr size_t FU_STATIC_LDS =
-    sizeof(int32_t) * (32 + FU_MAXT + FU_MAXL * FU_MAXL + 2 * FU_MAXL + 2 * FU_MAXM + 1) + FU_MAXS2D;
+    sizeof(int32_t) * (32 + FU_MAXT + FU_MAXL * FU_MAXL + 2 * FU_MAXL + FU_MAXM + 4) + FU_MAXS2D;
 
 // The dynamic LDS a block may take when `blocks` blocks share a compute unit.
 static size_t lds_limit(int blocks = 1) {
