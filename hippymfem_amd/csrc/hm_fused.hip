@@ -76,6 +76,11 @@ struct FusedGeom {
   int pn[2][4];
 };
 
+// Global memory (address space 1): loads through pointers read from LDS are global loads, not
+// flat ones (a flat load also counts as an LDS operation, so every later LDS wait would wait
+// for it).
+#define GLOBAL __attribute__((address_space(1)))
+
 // The type of the plane buffers' sums: FP64 for FP64 values (the bits of np.bincount); the
 // FP32 form sums in FP32, in the same ascending element order, so that its buffers take half
 // the LDS and two blocks share a compute unit.
@@ -144,6 +149,11 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
   __shared__ int32_t EFLAG[FU_MAXM];
   __shared__ int32_t TCOUNT[4];            // tasks of each sub-step group
   __shared__ uint8_t ROWOF[FU_MAXS2D];  // the row of every stencil place of a unit-height plane
+  // the element-data arrays by item kind (vertex, parameter dof, state dof): values (verts, m,
+  // u), p, and the element tables (elem_verts, edofs_m, edofs_u).  Read from LDS where a lane
+  // picks its item's array: held in scalar registers through the layer loop they spill, and a
+  // lane's choice between them costs a reload of each from the spill lanes.
+  __shared__ uint64_t PTAB[7];
 
   const int nu = P.nd_u, nm = P.nd_m;
   const bool fields = SP.need_fields != 0;
@@ -200,6 +210,15 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
       for (int i = tid; i < NQ3 * 24; i += BS) TX[(i % 24) * NQ3 + i / 24] = P.tab_x[i];
     for (int i = tid; i < NQ3 * nm; i += BS) TM[(i % nm) * NQ3 + i / nm] = P.tab_m[i];
     for (int i = tid; i < NQ3; i += BS) TW[i] = P.wq[i];
+    if (tid == 0) {
+      PTAB[0] = (uint64_t)P.verts;
+      PTAB[1] = (uint64_t)P.m;
+      PTAB[2] = (uint64_t)P.u;
+      PTAB[3] = (uint64_t)P.p;
+      PTAB[4] = (uint64_t)P.elem_verts;
+      PTAB[5] = (uint64_t)P.edofs_m;
+      PTAB[6] = (uint64_t)P.edofs_u;
+    }
     if (fields) {
       for (int i = tid; i < 2 * P.n1_u * NQ; i += BS) T1U[i] = P.tab1_u[i];
       for (int i = tid; i < nu; i += BS) UT[P.lex_u[i]] = i;
@@ -268,17 +287,9 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
   // into registers during the layer before (after its contraction, so that the loads are in
   // flight during its first adds and are not queued behind its stores) and put in LDS after
   // those adds, before the layer's planes are written out (a wait for the loads there waits for
-  // no store).  The arrays are read through pointers held in scalar registers: a per-lane choice
-  // of the problem's fields would read the pointers themselves from memory, one wait on every
-  // store before it.
+  // no store).  An item's array is chosen per lane from the pointer table in LDS (PTAB), with no
+  // branch: loads of divergent branches into one register would wait for each other.
   const int NI = 8 + nm + (fields ? nu : 0);
-  const int32_t* const ev_ = P.elem_verts;
-  const int32_t* const em_ = P.edofs_m;
-  const int32_t* const eu_ = P.edofs_u;
-  const double* const xv_ = P.verts;
-  const double* const mv_ = P.m;
-  const double* const uv_ = P.u;
-  const double* const pv_ = P.p;
   int32_t nid[FU_ITEMS];
   double nv[FU_ITEMS][3];
   auto item_elem = [&](int it, int ez) -> int64_t {
@@ -293,10 +304,10 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
       if (it >= M * NI || ez >= z1) continue;
       const int64_t e = item_elem(it, ez);
       const int w = it % NI;
-      // one load an item, its source chosen per lane (no branch: loads of divergent branches
-      // into one register would wait for each other)
-      const int32_t* src = w < 8 ? ev_ + (e * 8 + w) : w < 8 + nm ? em_ + (e * nm + (w - 8)) : eu_ + (e * nu + (w - 8 - nm));
-      nid[k] = *src;
+      const int kind = w < 8 ? 0 : w < 8 + nm ? 1 : 2;
+      const int nd = kind == 0 ? 8 : kind == 1 ? nm : nu;
+      const int sub = w - (kind == 0 ? 0 : kind == 1 ? 8 : 8 + nm);
+      nid[k] = ((const GLOBAL int32_t*)PTAB[4 + kind])[e * nd + sub];
     }
   };
   auto load_values = [&](int ez) {
@@ -308,14 +319,20 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
       const int64_t d = nid[k];
       // three loads an item, whatever its kind: a vertex's coordinates, a parameter dof's value
       // (three times), a state dof's u and p
-      const bool vx = w < 8, vm = !vx && w < 8 + nm;
-      const double* s0 = vx ? xv_ + d * 3 : vm ? mv_ + d : uv_ + d;
-      const double* s1 = vx ? s0 + 1 : vm ? s0 : pv_ + d;
-      const double* s2 = vx ? s0 + 2 : s0;
+      const int kind = w < 8 ? 0 : w < 8 + nm ? 1 : 2;
+      const GLOBAL double* s0 = (const GLOBAL double*)PTAB[kind] + d * (kind == 0 ? 3 : 1);
+      const GLOBAL double* s1 = kind == 0 ? s0 + 1 : kind == 1 ? s0 : (const GLOBAL double*)PTAB[3] + d;
+      const GLOBAL double* s2 = kind == 0 ? s0 + 2 : s0;
       nv[k][0] = *s0;
       nv[k][1] = *s1;
       nv[k][2] = *s2;
     }
+  };
+  // the element ids loaded a layer ahead waited for here, before the layer's stores: a wait for
+  // them after those stores (loads and stores complete in order) would wait for every store
+  auto settle_ids = [&]() {
+#pragma unroll
+    for (int k = 0; k < FU_ITEMS; ++k) asm volatile("" ::"v"(nid[k]));
   };
   auto store_values = [&](int ez) {
 #pragma unroll
@@ -452,12 +469,11 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
   load_values(zs);
   store_values(zs);
   load_ids(zs + 1);
+  settle_ids();
   for (int ez = zs; ez < z1; ++ez) {
     rederive();
-    // 1. this layer's element data, put in LDS by the layer before; the tables of the planes it
-    // completes in flight during the point phase
+    // 1. this layer's element data, put in LDS by the layer before
     __syncthreads();
-    load_planes(ez);
     // each element's alignment (every vertex coordinate that of the vertex with the same index
     // along its axis: J diagonal, the pairs of two different derivatives left out of its
     // contraction, as the element kernel does; lane v of the eight lanes of element m checks
@@ -494,7 +510,6 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
           ++p;
         }
     }
-    store_planes(ez);
     __syncthreads();
     // 3. the contraction of this thread's task
     rederive();
@@ -608,10 +623,12 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
         }
       }
     }
-    // the next layer's element data and planes' tables in flight during the adds
+    // the next layer's element data in flight during the adds
     rederive();
     load_values(ez + 1);
     load_ids(ez + 2);
+    // the tables of the planes this layer completes, in flight during the adds of phase A
+    load_planes(ez);
     // the task again, from LDS (nothing of it kept live through the contraction)
     has = tid < ntask;
     {
@@ -671,12 +688,14 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
               }
           }
         }
+        if (ph == 0 && g == 3) store_planes(ez);  // ordered before the write-out by the barrier
         __syncthreads();
       }
       // the next layer's element data into LDS (its loads had the adds to arrive; this layer's
       // point phase, the only reader of the region, is over), before this phase's stores
       if (!staged) {
         store_values(ez + 1);
+        settle_ids();
         staged = true;
       }
       // the planes of this phase no later layer adds to
@@ -697,7 +716,10 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
         }
       }
     }
-    if (!staged) store_values(ez + 1);
+    if (!staged) {
+      store_values(ez + 1);
+      settle_ids();
+    }
   }
   if (bad && P.nonfinite) atomicOr(P.nonfinite, 1);
 }
