@@ -114,7 +114,8 @@ __host__ inline FusedSizes fu_sizes(int bx, int by, int nslot, int nd_u, int nd_
   z.ew = (int)((((size_t)nslot * NQ3 * sizeof(CT) + 15) & ~(size_t)15) / sizeof(double)) + z.dw;
   // the tables; the trilinear map's gradients only where an element may not be axis-aligned
   z.table_words = (size_t)sf_table_words(nd_u, nd_m, fields, NQ) - (aligned ? 24 * NQ * NQ * NQ : 0) +
-                  4 * NT * NC * NQ;
+                  (((4 * NT * NC + (sizeof(CT) < 8 ? 2 * NT + 2 * NC : 0)) * NQ * sizeof(CT) + 15) & ~(size_t)15) /
+                      sizeof(double);
   z.elem_words = (size_t)z.m_max * z.ew;
   const size_t s2d = (size_t)z.sx_max * z.sy_max;
   // plane buffers in words of their sum type, 16 bytes whole, so that the staged maps after
@@ -184,13 +185,16 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
   const int64_t plane_rows = (int64_t)(pt * G.nx + 1) * (pt * G.ny + 1);
 
   // ------------------------------------------------------------------ LDS
-  // the tables: the parameter basis, the weights, the state's 1-D basis (fields), the 1-D table
-  // products, and (general form) the trilinear map's gradients
+  // the tables: the parameter basis, the weights, the state's 1-D basis (fields), (general form)
+  // the trilinear map's gradients, then in the contraction's type the 1-D table products PT and
+  // (FP32) the test and trial 1-D tables, read as they are used: no conversion in the contraction
   double* TM = lds;
   double* TW = TM + NQ3 * nm;
   double* T1U = TW + NQ3;
   double* TX = lds + sf_table_words(nu, nm, fields, NQ) - NQ3 * 24;
-  double* PT = AL ? TX : TX + NQ3 * 24;
+  CT* PT = (CT*)(AL ? TX : TX + NQ3 * 24);
+  CT* T1T = PT + 4 * NT * NC * NQ;
+  CT* T1C = T1T + 2 * NT * NQ;
   double* EL = lds + Z.table_words;                  // element regions
   BT* BUF0 = (BT*)(EL + Z.elem_words);               // plane buffers
   BT* BUF1 = BUF0 + Z.plane0_words;
@@ -204,7 +208,11 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
   {
     for (int i = tid; i < 4 * NT * NC * NQ; i += BS) {
       const int q = i % NQ, b = (i / NQ) % NC, a = (i / (NQ * NC)) % NT, dd = i / (NQ * NC * NT);
-      PT[i] = t1t[((dd >> 1) * NT + a) * NQ + q] * t1c[((dd & 1) * NC + b) * NQ + q];
+      PT[i] = (CT)(t1t[((dd >> 1) * NT + a) * NQ + q] * t1c[((dd & 1) * NC + b) * NQ + q]);
+    }
+    if constexpr (sizeof(CT) < 8) {
+      for (int i = tid; i < 2 * NT * NQ; i += BS) T1T[i] = (CT)t1t[i];
+      for (int i = tid; i < 2 * NC * NQ; i += BS) T1C[i] = (CT)t1c[i];
     }
     if constexpr (!AL)
       for (int i = tid; i < NQ3 * 24; i += BS) TX[(i % 24) * NQ3 + i / 24] = P.tab_x[i];
@@ -546,21 +554,21 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
           const int pw = G.pl[kind][t][k];
           const int d1 = pw & 3, d2 = (pw >> 2) & 3;
           const CT* wv = Wl + (pw >> 4) * NQ3;
-          const double* f1 = PT + ((d1 * NT + a1) * NC + b1) * NQ;
-          double gd[NQ];
+          const CT* f1 = PT + ((d1 * NT + a1) * NC + b1) * NQ;
+          CT gd[NQ];
 #pragma unroll
           for (int q = 0; q < NQ; ++q) gd[q] = f1[q];
-          double hd[NC][NQ];
+          CT hd[NC][NQ];
 #pragma unroll
           for (int i = 0; i < NC; ++i) {
-            const double* f2 = PT + ((d2 * NT + a2) * NC + i) * NQ;
+            const CT* f2 = PT + ((d2 * NT + a2) * NC + i) * NQ;
 #pragma unroll
             for (int q = 0; q < NQ; ++q) hd[i][q] = f2[q];
           }
           __builtin_amdgcn_sched_barrier(0);
           CT g[NQ];
 #pragma unroll
-          for (int q = 0; q < NQ; ++q) g[q] = (CT)gd[q];
+          for (int q = 0; q < NQ; ++q) g[q] = gd[q];
 #pragma unroll
           for (int q3 = 0; q3 < NQ; ++q3) {
             CT s1[NQ];
@@ -579,38 +587,39 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
             if constexpr (NQ == 3) {
 #pragma unroll
               for (int i = 0; i < NC; ++i)
-                s2[i][q3] = fma((CT)hd[i][2], s1[2], fma((CT)hd[i][1], s1[1], fma((CT)hd[i][0], s1[0], s2[i][q3])));
+                s2[i][q3] = fma(hd[i][2], s1[2], fma(hd[i][1], s1[1], fma(hd[i][0], s1[0], s2[i][q3])));
             } else {
 #pragma unroll
               for (int i = 0; i < NC; ++i) {
                 CT a = s2[i][q3];
 #pragma unroll
-                for (int q2 = 0; q2 < NQ; ++q2) a = fma((CT)hd[i][q2], s1[q2], a);
+                for (int q2 = 0; q2 < NQ; ++q2) a = fma(hd[i][q2], s1[q2], a);
                 s2[i][q3] = a;
               }
             }
           }
         }
-        const double* bt = t1t + (t >> 1) * NT * NQ;
-        const double* bc = t1c + (t & 1) * NC * NQ;
+        // (FP64: the kernel's arguments, read into scalar registers; FP32: the LDS copies)
+        const CT* bt = (std::is_same<CT, double>::value ? (const CT*)t1t : T1T) + (t >> 1) * NT * NQ;
+        const CT* bc = (std::is_same<CT, double>::value ? (const CT*)t1c : T1C) + (t & 1) * NC * NQ;
 #pragma unroll
         for (int b3 = 0; b3 < NC; ++b3) {
           CT u[NC][NQ];
 #pragma unroll
           for (int i = 0; i < NC; ++i)
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) u[i][q] = (CT)bc[b3 * NQ + q] * s2[i][q];
+            for (int q = 0; q < NQ; ++q) u[i][q] = bc[b3 * NQ + q] * s2[i][q];
 #pragma unroll
           for (int a3 = 0; a3 < NT; ++a3) {
             if constexpr (NQ == 3) {
-              const CT k0 = (CT)bt[a3 * 3], k1 = (CT)bt[a3 * 3 + 1], k2 = (CT)bt[a3 * 3 + 2];
+              const CT k0 = bt[a3 * 3], k1 = bt[a3 * 3 + 1], k2 = bt[a3 * 3 + 2];
 #pragma unroll
               for (int i = 0; i < NC; ++i)
                 acc[i][a3][b3] = fma(k2, u[i][2], fma(k1, u[i][1], fma(k0, u[i][0], acc[i][a3][b3])));
             } else {
               CT k[NQ];
 #pragma unroll
-              for (int q = 0; q < NQ; ++q) k[q] = (CT)bt[a3 * NQ + q];
+              for (int q = 0; q < NQ; ++q) k[q] = bt[a3 * NQ + q];
 #pragma unroll
               for (int i = 0; i < NC; ++i) {
                 CT a = acc[i][a3][b3];
