@@ -435,6 +435,7 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
     const int S = s2d_row * wz;
     const uint32_t mul = (65536u + (uint32_t)wz - 1u) / (uint32_t)wz;
     BT* buf = (lz & 1) ? BUF1 : BUF0;
+    const BT diagv = (BT)G.diag;
     const uint8_t* sg = SIG + k * Z.sig_bytes;
     const int2* hdr = HDR + k * Z.rows_max;
     for (int base = 0; base < S; base += FU_EB * BS) {
@@ -463,8 +464,9 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
         if (kk[e] < S) {
           buf[rb[e] + (c[e] & 0x7F)] = 0;
           const int t = kk[e] - rb[e];
-          double o = (c[e] & 0x80) ? 0.0 : (double)v[e];
-          if (hd[e].y != 0) o = (hd[e].y == 2 && t == 0) ? G.diag : 0.0;
+          // (in the sums' type: the FP32 form converts nothing here)
+          BT o = (c[e] & 0x80) ? (BT)0 : v[e];
+          if (hd[e].y != 0) o = (hd[e].y == 2 && t == 0) ? diagv : (BT)0;
           data[(int64_t)hd[e].x + t] = (OT)o;
         }
     }
