@@ -764,8 +764,10 @@ static FusedSizes fu_pick(const hm_problem& P, const SumfactPlan& SP, int nslot,
 }
 
 // the static LDS of k_fused (task table, row places, widths, flags), left out of its dynamic LDS
+// (with 256 bytes for the alignment of the arrays)
 constexpr size_t FU_STATIC_LDS =
-    sizeof(int32_t) * (32 + FU_MAXT + FU_MAXL * FU_MAXL + 2 * FU_MAXL + FU_MAXM + 4) + FU_MAXS2D;
+    sizeof(int32_t) * (32 + FU_MAXT + FU_MAXL * FU_MAXL + 2 * FU_MAXL + FU_MAXM + 4) + FU_MAXS2D +
+    7 * sizeof(uint64_t) + 256;
 
 // The dynamic LDS a block may take when `blocks` blocks share a compute unit.
 static size_t lds_limit(int blocks = 1) {
