@@ -64,6 +64,28 @@ __device__ inline void geom_from_J(const double J[3][3], GeomT<PT>& G) {
       G.K[k][l] = G.Ji[k][0] * G.Ji[l][0] + G.Ji[k][1] * G.Ji[l][1] + G.Ji[k][2] * G.Ji[l][2];
 }
 
+// The geometry of an axis-aligned box element, J = diag(h0, h1, h2) (the edge lengths): the
+// diagonal terms of geom_from_J's cofactor formulas in FP64, the other terms zero.  The element's
+// geometry once, as the reference keeps one J^-1 an element for affine groups
+// (hippymfem/fem/elementbatch.py:206-223), instead of the trilinear map's multiply-adds and the
+// full inverse at every point.
+template <typename PT>
+__device__ inline void geom_diag(double h0, double h1, double h2, GeomT<PT>& G) {
+  const double c0 = h1 * h2, c1 = h0 * h2, c2 = h0 * h1;
+  const double det = h0 * c0;
+  const double id = 1.0 / det;
+  for (int k = 0; k < 3; ++k)
+    for (int l = 0; l < 3; ++l) {
+      G.Ji[k][l] = (PT)0;
+      G.K[k][l] = (PT)0;
+    }
+  G.Ji[0][0] = (PT)(c0 * id);
+  G.Ji[1][1] = (PT)(c1 * id);
+  G.Ji[2][2] = (PT)(c2 * id);
+  G.adet = fabs(det);
+  for (int k = 0; k < 3; ++k) G.K[k][k] = G.Ji[k][k] * G.Ji[k][k];
+}
+
 // The point state from the geometry at the point, the point's quadrature weight and the
 // fields' values and reference gradients there, all after the geometry in PT.
 template <typename PT>

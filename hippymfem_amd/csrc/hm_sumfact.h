@@ -77,19 +77,6 @@ __host__ __device__ inline int sf_element_words(int npair, int nd_u, int nd_m, i
 // PT: the point type, FP64, or FP32 in the precision forms: the fields, their gradients and
 // everything after the geometry are then FP32 (the dof values and tables rounded as read), the
 // geometry FP64 (point_finish).
-// The J of an axis-aligned box element (every vertex coordinate that of the vertex with the same
-// index along its axis; the caller tests it once an element): the constant diagonal of its edge
-// lengths, the element's geometry once, as the reference keeps one J^-1 an element for affine
-// groups (hippymfem/fem/elementbatch.py:206-223), instead of the trilinear map's 72
-// multiply-adds at every point.
-__device__ inline void sf_aligned_J(const double* X, double J[3][3]) {
-  for (int d = 0; d < 3; ++d)
-    for (int k = 0; k < 3; ++k) J[d][k] = 0.0;
-  J[0][0] = X[1 * 3 + 0] - X[0];
-  J[1][1] = X[2 * 3 + 1] - X[1];
-  J[2][2] = X[4 * 3 + 2] - X[2];
-}
-
 // The fields at point q: u and its reference gradient, p's, m.
 template <int NQ, typename PT>
 __device__ inline void sf_fields(const hm_problem& P, int q, const double* U, const double* M, const double* Pv,
@@ -141,10 +128,14 @@ __device__ inline void sf_point_state(const hm_problem& P, int q, const double* 
                                       const double* M, const double* Pv, bool fields, const double* TX,
                                       const double* TM, const double* TW, const double* T1U, PointStateT<PT>& S,
                                       bool aligned = false) {
-  double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  GeomT<PT> G;
   if (aligned) {
-    sf_aligned_J(X, J);
+    // an axis-aligned box element (every vertex coordinate that of the vertex with the same
+    // index along its axis; the caller tests it once an element): J the diagonal of its edge
+    // lengths
+    geom_diag<PT>(X[1 * 3 + 0] - X[0], X[2 * 3 + 1] - X[1], X[4 * 3 + 2] - X[2], G);
   } else {
+    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
 #pragma unroll 2
     for (int v = 0; v < 8; ++v)
 #pragma unroll
@@ -154,10 +145,11 @@ __device__ inline void sf_point_state(const hm_problem& P, int q, const double* 
         J[1][k] = fma(X[v * 3 + 1], g, J[1][k]);
         J[2][k] = fma(X[v * 3 + 2], g, J[2][k]);
       }
+    geom_from_J<PT>(J, G);
   }
   PT uq, mq, ru[3], rp[3];
   sf_fields<NQ, PT>(P, q, U, M, Pv, fields, TM, T1U, uq, ru, rp, mq);
-  point_finish<PT>(P, TW[q], J, uq, ru, rp, mq, S);
+  point_fields<PT>(P, TW[q], G, uq, ru, rp, mq, S);
 }
 
 // LDS words of the tables every element reads (nq points, nq1 a direction): the trilinear
