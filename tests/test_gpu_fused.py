@@ -181,3 +181,22 @@ def test_perturbed_mesh_takes_general_form(gpu):
     prob = PDEVariationalProblem(W.spaces, W.density, W.batches, W.bc)
     f = prob.block_assembly(ADJOINT, STATE, route="fused")
     assert f.fused_plan().struct.flags == 0
+
+
+def test_fp32_route_choice_and_blocks_a_compute_unit(gpu):
+    """``hm_fused_blocks``: the FP32 form of A on a box mesh runs two blocks a compute unit
+    (its tile's LDS fits half the unit's), FP64 and the perturbed mesh's FP32 form one; the auto
+    route takes the fused kernel for FP32 exactly where it runs two."""
+    from hippymfem_amd import fused
+
+    box = Workload(8)
+    pert = Workload(8, perturb=0.2)
+    for W, aligned in ((box, True), (pert, False)):
+        prob = PDEVariationalProblem(W.spaces, W.density, W.batches, W.bc)
+        k = prob.kernel
+        assert fused.aligned_box(W.mesh) is aligned
+        assert fused.blocks_of(k, ADJOINT, STATE, "fp64", aligned) == 1
+        nb = fused.blocks_of(k, ADJOINT, STATE, "fp32", aligned)
+        assert nb == (2 if aligned else 1)
+        auto = prob.block_assembly(ADJOINT, STATE, "fp32")
+        assert (auto.fused_plan() is not None) == (nb == 2), auto.why_two_pass
