@@ -573,10 +573,15 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
           for (int q = 0; q < NQ; ++q) g[q] = gd[q];
 #pragma unroll
           for (int q3 = 0; q3 < NQ; ++q3) {
+            // the slice's NQ^2 pair values read together: one LDS round trip a slice
+            CT wq[NQ2];
+#pragma unroll
+            for (int r = 0; r < NQ2; ++r) wq[r] = wv[NQ2 * q3 + r];
+            __builtin_amdgcn_sched_barrier(0);
             CT s1[NQ];
 #pragma unroll
             for (int q2 = 0; q2 < NQ; ++q2) {
-              const CT* wr = wv + NQ * q2 + NQ2 * q3;
+              const CT* wr = wq + NQ * q2;
               if constexpr (NQ == 3) {
                 s1[q2] = fma(g[2], wr[2], fma(g[1], wr[1], g[0] * wr[0]));
               } else {
