@@ -155,6 +155,7 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
   // picks its item's array: held in scalar registers through the layer loop they spill, and a
   // lane's choice between them costs a reload of each from the spill lanes.
   __shared__ uint64_t PTAB[7];
+  __shared__ int32_t EOFF[FU_MAXM];  // every element slot's element within its layer, ex + n_x ey
 
   const int nu = P.nd_u, nm = P.nd_m;
   const bool fields = SP.need_fields != 0;
@@ -227,6 +228,7 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
       PTAB[5] = (uint64_t)P.edofs_m;
       PTAB[6] = (uint64_t)P.edofs_u;
     }
+    for (int m = tid; m < M; m += BS) EOFF[m] = (xs + m % cx) + G.nx * (ys + m / cx);
     if (fields) {
       for (int i = tid; i < 2 * P.n1_u * NQ; i += BS) T1U[i] = P.tab1_u[i];
       for (int i = tid; i < nu; i += BS) UT[P.lex_u[i]] = i;
@@ -300,18 +302,20 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
   const int NI = 8 + nm + (fields ? nu : 0);
   int32_t nid[FU_ITEMS];
   double nv[FU_ITEMS][3];
-  auto item_elem = [&](int it, int ez) -> int64_t {
-    const int m = it / NI;
-    const int ex = xs + m % cx, ey = ys + m / cx;
-    return (int64_t)ex + (int64_t)G.nx * ((int64_t)ey + (int64_t)G.ny * ez);
-  };
+  const int64_t lstride = (int64_t)G.nx * G.ny;  // elements a layer
+  // an item's element slot it / NI by a rounded FP32 product with 1 / NI: exact for it < 2^13 and
+  // NI <= 128 (every pair checked), fewer operations than an integer division
+  const float rNI = 1.0f / (float)NI;
+  auto item_slot = [&](int it) { return (int)__fmul_rn((float)it + 0.5f, rNI); };
+  auto item_elem = [&](int m, int ez) -> int64_t { return (int64_t)EOFF[m] + lstride * ez; };
   auto load_ids = [&](int ez) {
 #pragma unroll
     for (int k = 0; k < FU_ITEMS; ++k) {
       const int it = tid + k * BS;
       if (it >= M * NI || ez >= z1) continue;
-      const int64_t e = item_elem(it, ez);
-      const int w = it % NI;
+      const int m = item_slot(it);
+      const int64_t e = item_elem(m, ez);
+      const int w = it - m * NI;
       const int kind = w < 8 ? 0 : w < 8 + nm ? 1 : 2;
       const int nd = kind == 0 ? 8 : kind == 1 ? nm : nu;
       const int sub = w - (kind == 0 ? 0 : kind == 1 ? 8 : 8 + nm);
@@ -323,7 +327,7 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
     for (int k = 0; k < FU_ITEMS; ++k) {
       const int it = tid + k * BS;
       if (it >= M * NI || ez >= z1) continue;
-      const int w = it % NI;
+      const int w = it - item_slot(it) * NI;
       const int64_t d = nid[k];
       // three loads an item, whatever its kind: a vertex's coordinates, a parameter dof's value
       // (three times), a state dof's u and p
@@ -347,7 +351,7 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
     for (int k = 0; k < FU_ITEMS; ++k) {
       const int it = tid + k * BS;
       if (it >= M * NI || ez >= z1) continue;
-      const int m = it / NI, w = it % NI;
+      const int m = item_slot(it), w = it - m * NI;
       double* D = Dreg(m);
       if (w < 8) {
         D[w * 3] = nv[k][0];
@@ -771,7 +775,7 @@ static FusedSizes fu_pick(const hm_problem& P, const SumfactPlan& SP, int nslot,
 // the static LDS of k_fused (task table, row places, widths, flags), left out of its dynamic LDS
 // (with 256 bytes for the alignment of the arrays)
 constexpr size_t FU_STATIC_LDS =
-    sizeof(int32_t) * (32 + FU_MAXT + FU_MAXL * FU_MAXL + 2 * FU_MAXL + FU_MAXM + 4) + FU_MAXS2D +
+    sizeof(int32_t) * (32 + FU_MAXT + FU_MAXL * FU_MAXL + 2 * FU_MAXL + 2 * FU_MAXM + 4) + FU_MAXS2D +
     7 * sizeof(uint64_t) + 256;
 
 // The dynamic LDS a block may take when `blocks` blocks share a compute unit.
@@ -1024,6 +1028,7 @@ extern "C" int hm_assemble_fused(const hm_problem* P, int i, int j, const hm_fus
   HM_REQUIRE(F->nx >= 1 && F->ny >= 1 && F->nz >= 1 && (int64_t)F->nx * F->ny * F->nz == P->ne,
              "a box of %d x %d x %d elements against %lld", F->nx, F->ny, F->nz, (long long)P->ne);
   HM_REQUIRE(F->hdr && F->hdr_off && F->sigma && F->sig_off, "the fused plan is incomplete");
+  HM_REQUIRE((int64_t)F->nx * F->ny < ((int64_t)1 << 31), "a layer of %d x %d elements", F->nx, F->ny);
   SumfactPlan SP;
   int nt = 0, nc = 0;
   if (int rc = fu_plan(P, i, j, SP, &nt, &nc)) return rc;
