@@ -303,8 +303,9 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
   int32_t nid[FU_ITEMS];
   double nv[FU_ITEMS][3];
   const int64_t lstride = (int64_t)G.nx * G.ny;  // elements a layer
-  // an item's element slot it / NI by a rounded FP32 product with 1 / NI: exact for it < 2^13 and
-  // NI <= 128 (every pair checked), fewer operations than an integer division
+  // an item's element slot it / NI by a rounded FP32 product with 1 / NI, fewer operations than an
+  // integer division: exact for it < 2^13 and NI <= 128 (the product is within 2^-10 of
+  // (it + 1/2) / NI, whose distance to an integer is at least 1 / (2 NI))
   const float rNI = 1.0f / (float)NI;
   auto item_slot = [&](int it) { return (int)__fmul_rn((float)it + 0.5f, rNI); };
   auto item_elem = [&](int m, int ez) -> int64_t { return (int64_t)EOFF[m] + lstride * ez; };

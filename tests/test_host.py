@@ -189,3 +189,16 @@ def test_scatter_refuses_rows_beyond_its_buffers(nd_c, longest, lanes):
     assert "row of %d entries" % longest in msg and str(256 // lanes) in msg
     with pytest.raises(NotImplementedError):
         hm_lib.check(rc, "scatter")
+
+
+def test_fused_item_slot_division_is_exact():
+    """The fused kernel takes an element-data item's slot it // NI as the truncated FP32 product
+    (it + 1/2) * (1 / NI), both roundings to nearest; exact over the kernel's range (it below
+    2^13, NI at most 128), checked here in the same FP32 arithmetic."""
+    import numpy as np
+
+    it = np.arange(1 << 13, dtype=np.int64)
+    for ni in range(1, 129):
+        r = np.float32(1.0) / np.float32(ni)
+        got = ((it.astype(np.float32) + np.float32(0.5)) * r).astype(np.int64)
+        assert np.array_equal(got, it // ni), ni
