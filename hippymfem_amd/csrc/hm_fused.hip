@@ -304,8 +304,9 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
   double nv[FU_ITEMS][3];
   const int64_t lstride = (int64_t)G.nx * G.ny;  // elements a layer
   // an item's element slot it / NI by a rounded FP32 product with 1 / NI, fewer operations than an
-  // integer division: exact for it < 2^13 and NI <= 128 (the product is within 2^-10 of
-  // (it + 1/2) / NI, whose distance to an integer is at least 1 / (2 NI))
+  // integer division: exact for it < 2^13 and NI <= 128 (the product is within 2^-9 of
+  // (it + 1/2) / NI, even with 1 / NI a unit in the last place off, and (it + 1/2) / NI is at
+  // least 1 / (2 NI) >= 2^-8 from an integer)
   const float rNI = 1.0f / (float)NI;
   auto item_slot = [&](int it) { return (int)__fmul_rn((float)it + 0.5f, rNI); };
   auto item_elem = [&](int m, int ez) -> int64_t { return (int64_t)EOFF[m] + lstride * ez; };
