@@ -89,8 +89,8 @@ template <> struct FuSum<float> { using T = float; };
 
 // Sizes of a tile's LDS, for the launch and the host's choice of tile.
 struct FusedSizes {
-  int m_max, t_max, rows_max, sx_max, sy_max, dw, ew;
-  size_t table_words, elem_words, plane0_words, plane1_words, sig_bytes, bytes;
+  int m_max, t_max, rows_max, sx_max, sy_max, dw, pw;
+  size_t table_words, wa_words, elem_words, plane0_words, plane1_words, sig_bytes, bytes;
 };
 
 // The stencil width a direction of a lattice point l of order pt (n elements), trial order pc.
@@ -109,24 +109,31 @@ __host__ inline FusedSizes fu_sizes(int bx, int by, int nslot, int nd_u, int nd_
   z.rows_max = (pt * bx + 1) * (pt * by + 1);
   z.sx_max = bx * (2 * pc + 1) + (pt - 1) * bx * (pc + 1) + (pc + 1);
   z.sy_max = by * (2 * pc + 1) + (pt - 1) * by * (pc + 1) + (pc + 1);
-  z.dw = (24 + nd_m + (fields ? 2 * nd_u : 0) + 1) & ~1;
-  // an element's pairs in the contraction's type (words of FP64, even), then its data
-  z.ew = (int)((((size_t)nslot * NQ3 * sizeof(CT) + 15) & ~(size_t)15) / sizeof(double)) + z.dw;
+  // an element slot's data (words of FP64): 2 mod 4 words, so that the slots' regions start on
+  // 16 different LDS banks
+  z.dw = ((24 + nd_m + (fields ? 2 * nd_u : 0) + 1) & ~1) | 2;
+  // an element slot's pairs in the contraction's type (words of FP64, even)
+  z.pw = (int)((((size_t)nslot * NQ3 * sizeof(CT) + 15) & ~(size_t)15) / sizeof(double));
   // the tables; the trilinear map's gradients only where an element may not be axis-aligned
   z.table_words = (size_t)sf_table_words(nd_u, nd_m, fields, NQ) - (aligned ? 24 * NQ * NQ * NQ : 0) +
                   (((4 * NT * NC + (sizeof(CT) < 8 ? 2 * NT + 2 * NC : 0)) * NQ * sizeof(CT) + 15) & ~(size_t)15) /
                       sizeof(double);
-  z.elem_words = (size_t)z.m_max * z.ew;
   const size_t s2d = (size_t)z.sx_max * z.sy_max;
+  z.sig_bytes = ((s2d * (2 * pc + 1) + 15) & ~(size_t)15);
+  // the pairs of every element slot, and in the same words, once the contraction is over, the
+  // staged stencil maps and row headers of FU_SLOTS planes; then every slot's data
+  const size_t stage_words = ((FU_SLOTS * z.sig_bytes + FU_SLOTS * (size_t)z.rows_max * sizeof(int2) + 15) & ~(size_t)15) /
+                             sizeof(double);
+  z.wa_words = std::max((size_t)z.m_max * z.pw, stage_words);
+  z.elem_words = z.wa_words + (size_t)z.m_max * z.dw;
   // plane buffers in words of their sum type, 16 bytes whole, so that the staged maps after
   // them start 16-byte aligned
   using BT = typename FuSum<CT>::T;
   constexpr size_t A16 = 16 / sizeof(BT);
   z.plane0_words = (s2d * (2 * pc + 1) + A16 - 1) & ~(A16 - 1);
   z.plane1_words = (s2d * (pt == 2 ? pc + 1 : 2 * pc + 1) + A16 - 1) & ~(A16 - 1);
-  z.sig_bytes = ((s2d * (2 * pc + 1) + 15) & ~(size_t)15);
   z.bytes = (z.table_words + z.elem_words) * sizeof(double) + (z.plane0_words + z.plane1_words) * sizeof(BT) +
-            FU_SLOTS * z.sig_bytes + FU_SLOTS * (size_t)z.rows_max * sizeof(int2) + (size_t)(pt * nz_max + 1) * 8;
+            (size_t)(pt * nz_max + 1) * 8;
   return z;
 }
 
@@ -196,15 +203,20 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
   CT* PT = (CT*)(AL ? TX : TX + NQ3 * 24);
   CT* T1T = PT + 4 * NT * NC * NQ;
   CT* T1C = T1T + 2 * NT * NQ;
-  double* EL = lds + Z.table_words;                  // element regions
-  BT* BUF0 = (BT*)(EL + Z.elem_words);               // plane buffers
+  // The element slots' pairs share their words with the staged tables of the planes a layer
+  // writes out: the pairs are read by the contraction only, and every wavefront's contraction is
+  // over at the first sub-step barrier of the adds; the tables are staged at the last sub-step of
+  // phase A and read by the write-outs, all before the next layer's top barrier, after which its
+  // point phase writes the pairs again.
+  double* WA = lds + Z.table_words;                  // pairs of the element slots
+  double* DA = WA + Z.wa_words;                      // data of the element slots
+  BT* BUF0 = (BT*)(WA + Z.elem_words);               // plane buffers
   BT* BUF1 = BUF0 + Z.plane0_words;
-  uint8_t* SIG = (uint8_t*)(BUF1 + Z.plane1_words);  // staged stencil maps, FU_SLOTS planes
+  uint8_t* SIG = (uint8_t*)WA;                       // staged stencil maps, FU_SLOTS planes
   int2* HDR = (int2*)(SIG + FU_SLOTS * Z.sig_bytes); // staged row headers, FU_SLOTS planes
-  int64_t* SOFF = (int64_t*)(HDR + FU_SLOTS * Z.rows_max);  // the chunk's planes' map starts
-  const int wpair = (int)((((size_t)G.nslot * NQ3 * sizeof(CT) + 15) & ~(size_t)15) / sizeof(double));
-  auto Wreg = [&](int m) { return (CT*)(EL + (size_t)m * Z.ew); };
-  auto Dreg = [&](int m) { return EL + (size_t)m * Z.ew + wpair; };
+  int64_t* SOFF = (int64_t*)(BUF1 + Z.plane1_words); // the chunk's planes' map starts
+  auto Wreg = [&](int m) { return (CT*)(WA + (size_t)m * Z.pw); };
+  auto Dreg = [&](int m) { return DA + (size_t)m * Z.dw; };
 
   {
     for (int i = tid; i < 4 * NT * NC * NQ; i += BS) {

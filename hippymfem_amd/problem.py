@@ -185,15 +185,13 @@ class BlockAssembly:
         return want <= 0.5 * torch.cuda.mem_get_info(self.data.device)[0]
 
     def _fp32_two_blocks(self):
-        """Whether the fused kernel of this block runs two blocks a compute unit (its FP32 form
-        on a mesh of axis-aligned boxes, where the tile's LDS fits half the unit's)."""
+        """Whether the fused kernel of this block runs two blocks a compute unit (its FP32 form,
+        where the tile's LDS fits half the unit's, in the form of the mesh's kind)."""
         if self.precision != "fp32" or fused.supported(self.kernel, self.i, self.j,
                                                        self.precision) is not None:
             return False
-        mesh = self.kernel.batches.mesh
-        if not fused.aligned_box(mesh):
-            return False
-        return fused.blocks_of(self.kernel, self.i, self.j, self.precision, True) == 2
+        aligned = fused.aligned_box(self.kernel.batches.mesh)
+        return fused.blocks_of(self.kernel, self.i, self.j, self.precision, aligned) == 2
 
     @property
     def dplan(self):
@@ -228,13 +226,14 @@ class BlockAssembly:
                     not self._fp32_two_blocks():
                 # single precision: FP32 element matrices halve the two-pass route's traffic;
                 # the fused kernel gains on it only where two of its blocks share a compute
-                # unit, on box meshes (64^3 P2 A: 0.68 ms fused against 0.92 ms two-pass; with
-                # one block, perturbed: 1.26 ms fused against 1.15 ms two-pass)
+                # unit (64^3 P2 A: box 0.56 ms fused against 0.83 ms two-pass, perturbed 0.85
+                # against 1.05; with one block, perturbed: 1.10 ms fused)
                 why = "single precision with one fused block a compute unit (the two-pass " \
                       "route is faster)"
             elif self.route == "auto" and self.kernel.group.nq1 > 3:
                 # the 4-point rule: 64 points' pairs in LDS leave the fused kernel small tiles
-                # (64^3 P2 A, degree 6: 2.25 ms fused against 1.71 ms two-pass)
+                # and blocks of 512 threads (64^3 P2 A, degree 6: 2.01 ms fused on 4 x 3 tiles
+                # against 1.38 ms two-pass)
                 why = "a 4-point rule (the two-pass route is faster)"
             elif self.route == "auto" and self.kernel.spaces[self.j].order < 2:
                 # a Q1 trial space: the element matrices are small against the point work

@@ -103,8 +103,8 @@ def fp32_sums_in_element_order(t):
 def test_fused_fp32_sums_in_fp32_in_element_order(gpu, n, nonlinear, perturb):
     """The FP32 form sums its FP32 entries in FP32, in ascending element order: its values are
     the host's FP32 sums of the element kernel's entries in that order, bit for bit, on
-    perturbed meshes (one block a compute unit) and box meshes (two, the 9^3 case in chunks of
-    layers); and within 1e-6 of the FP64 values."""
+    perturbed meshes and box meshes (two blocks a compute unit, the 9^3 case in chunks of layers;
+    the perturbed nonlinear blocks of many pair slots one); and within 1e-6 of the FP64 values."""
     W = Workload(n, nonlinear=nonlinear, perturb=perturb)
     prob = PDEVariationalProblem(W.spaces, W.density, W.batches, W.bc, is_fwd_linear=not nonlinear)
     loc = [torch.as_tensor(v).cuda() for v in (W.u, W.m, W.p)]
@@ -184,19 +184,27 @@ def test_perturbed_mesh_takes_general_form(gpu):
 
 
 def test_fp32_route_choice_and_blocks_a_compute_unit(gpu):
-    """``hm_fused_blocks``: the FP32 form of A on a box mesh runs two blocks a compute unit
-    (its tile's LDS fits half the unit's), FP64 and the perturbed mesh's FP32 form one; the auto
-    route takes the fused kernel for FP32 exactly where it runs two."""
+    """``hm_fused_blocks``: the FP32 form of A runs two blocks a compute unit on box and perturbed
+    meshes (its tile's LDS fits half the unit's), FP64 one; the nonlinear blocks of the perturbed
+    mesh hold more pair slots and some run one. The auto route takes the fused kernel for FP32
+    exactly where it runs two."""
     from hippymfem_amd import fused
 
-    box = Workload(8)
-    pert = Workload(8, perturb=0.2)
-    for W, aligned in ((box, True), (pert, False)):
-        prob = PDEVariationalProblem(W.spaces, W.density, W.batches, W.bc)
+    seen = set()
+    for n, nonlinear, perturb in ((8, False, 0.0), (8, False, 0.2), (6, True, 0.2)):
+        W = Workload(n, nonlinear=nonlinear, perturb=perturb)
+        prob = PDEVariationalProblem(W.spaces, W.density, W.batches, W.bc, is_fwd_linear=not nonlinear)
         k = prob.kernel
+        aligned = perturb == 0.0
         assert fused.aligned_box(W.mesh) is aligned
-        assert fused.blocks_of(k, ADJOINT, STATE, "fp64", aligned) == 1
-        nb = fused.blocks_of(k, ADJOINT, STATE, "fp32", aligned)
-        assert nb == (2 if aligned else 1)
-        auto = prob.block_assembly(ADJOINT, STATE, "fp32")
-        assert (auto.fused_plan() is not None) == (nb == 2), auto.why_two_pass
+        for ij in BLOCKS + ([(STATE, STATE)] if nonlinear else []):
+            if ij[1] == PARAMETER:
+                continue                       # Q1 trial spaces: the two-pass route
+            assert fused.blocks_of(k, ij[0], ij[1], "fp64", aligned) == 1
+            nb = fused.blocks_of(k, ij[0], ij[1], "fp32", aligned)
+            seen.add(nb)
+            if not nonlinear and ij == (ADJOINT, STATE):
+                assert nb == 2, (n, perturb)
+            auto = prob.block_assembly(ij[0], ij[1], "fp32")
+            assert (auto.fused_plan() is not None) == (nb == 2), (ij, auto.why_two_pass)
+    assert seen == {1, 2}
