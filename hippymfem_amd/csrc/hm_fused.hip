@@ -306,11 +306,12 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
   // ------------------------------------------------------------------ element data
   // items of a layer: per element slot 8 vertices, nd_m parameter dofs, and (fields) nd_u
   // state/adjoint dofs; thread tid takes items tid, tid + BS, ...  A layer's data is loaded
-  // into registers during the layer before (after its contraction, so that the loads are in
-  // flight during its first adds and are not queued behind its stores) and put in LDS after
-  // those adds, before the layer's planes are written out (a wait for the loads there waits for
-  // no store).  An item's array is chosen per lane from the pointer table in LDS (PTAB), with no
-  // branch: loads of divergent branches into one register would wait for each other.
+  // into registers during the layer before (before its contraction, so that the loads are in
+  // flight during the contraction and the first adds and are not queued behind its stores) and
+  // put in LDS after those adds, before the layer's planes are written out (a wait for the loads
+  // there waits for no store).  An item's array is chosen per lane from the pointer table in LDS
+  // (PTAB), with no branch: loads of divergent branches into one register would wait for each
+  // other.
   const int NI = 8 + nm + (fields ? nu : 0);
   int32_t nid[FU_ITEMS];
   double nv[FU_ITEMS][3];
@@ -539,8 +540,10 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
         }
     }
     __syncthreads();
-    // 3. the contraction of this thread's task
+    // 3. the contraction of this thread's task, the next layer's element data in flight
     rederive();
+    load_values(ez + 1);
+    load_ids(ez + 2);
     bool has = tid < ntask;
     int m = 0, a1 = 0, b1 = 0, a2 = 0;
     if (has) {
@@ -657,10 +660,7 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
         }
       }
     }
-    // the next layer's element data in flight during the adds
     rederive();
-    load_values(ez + 1);
-    load_ids(ez + 2);
     // the tables of the planes this layer completes, in flight during the adds of phase A
     load_planes(ez);
     // the task again, from LDS (nothing of it kept live through the contraction)
