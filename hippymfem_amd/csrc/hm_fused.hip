@@ -307,9 +307,9 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
   // items of a layer: per element slot 8 vertices, nd_m parameter dofs, and (fields) nd_u
   // state/adjoint dofs; thread tid takes items tid, tid + BS, ...  A layer's data is loaded
   // into registers during the layer before (before its contraction, so that the loads are in
-  // flight during the contraction and the first adds and are not queued behind its stores) and
-  // put in LDS after those adds, before the layer's planes are written out (a wait for the loads
-  // there waits for no store).  An item's array is chosen per lane from the pointer table in LDS
+  // flight during the contraction and are not queued behind its stores) and put in LDS after the
+  // contraction (aligned form) or after the first adds (general form), before the layer's planes
+  // are written out (a wait for the loads there waits for no store).  An item's array is chosen per lane from the pointer table in LDS
   // (PTAB), with no branch: loads of divergent branches into one register would wait for each
   // other.
   const int NI = 8 + nm + (fields ? nu : 0);
@@ -661,6 +661,13 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
       }
     }
     rederive();
+    // the next layer's element data into LDS (the point phase, the only reader of the region, is
+    // over, and the loads had the contraction to arrive).  The general form keeps part of the
+    // loaded values in scratch memory over the contraction, and stores them after the first adds.
+    if constexpr (AL) {
+      store_values(ez + 1);
+      settle_ids();
+    }
     // the tables of the planes this layer completes, in flight during the adds of phase A
     load_planes(ez);
     // the task again, from LDS (nothing of it kept live through the contraction)
@@ -685,7 +692,7 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
     const int wx = has ? WXs[ix] : 0, wy = has ? WYs[iy] : 0;
     const int b2d = has ? B2D[iy * cxl + ix] : 0;
     int rk = 0;            // staged planes written so far
-    bool staged = false;   // the next layer's element data in LDS
+    bool staged = AL;      // the next layer's element data in LDS
     constexpr int NA = NT < 2 ? NT : 2;  // rows of phase A: a3 < NA
 #pragma unroll
     for (int ph = 0; ph < 2; ++ph) {
