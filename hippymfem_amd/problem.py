@@ -230,6 +230,15 @@ class BlockAssembly:
                 # against 1.05; with one block, perturbed: 1.10 ms fused)
                 why = "single precision with one fused block a compute unit (the two-pass " \
                       "route is faster)"
+            elif self.route == "auto" and self.precision == "fp64" and \
+                    getattr(self.kernel.density, "nonlinear", False) and \
+                    not fused.aligned_box(self.kernel.batches.mesh):
+                # a nonlinear density on a mesh of non-aligned elements: the fields' dofs in
+                # the element data and the general form's pair slots leave the fused kernel
+                # smaller tiles (96^3 perturbed: A 6.05 ms fused against 5.10 ms two-pass,
+                # W_uu 5.29 against 4.47; on box meshes the fused route is faster)
+                why = "a nonlinear density on a mesh of non-aligned elements (the two-pass " \
+                      "route is faster)"
             elif self.route == "auto" and self.kernel.group.nq1 > 3:
                 # the 4-point rule: 64 points' pairs in LDS leave the fused kernel small tiles
                 # and blocks of 512 threads (64^3 P2 A, degree 6: 2.01 ms fused on 4 x 3 tiles

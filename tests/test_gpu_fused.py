@@ -208,3 +208,18 @@ def test_fp32_route_choice_and_blocks_a_compute_unit(gpu):
             auto = prob.block_assembly(ij[0], ij[1], "fp32")
             assert (auto.fused_plan() is not None) == (nb == 2), (ij, auto.why_two_pass)
     assert seen == {1, 2}
+
+
+def test_fp64_auto_route_by_density_and_mesh(gpu):
+    """The auto route of the FP64 Q2 blocks: fused on box meshes (either density) and for the
+    linear density on perturbed meshes; two-pass for the nonlinear density on perturbed meshes,
+    where the fused kernel's tiles are smaller."""
+    for nonlinear, perturb, want in ((False, 0.0, True), (False, 0.2, True), (True, 0.0, True),
+                                     (True, 0.2, False)):
+        W = Workload(6, nonlinear=nonlinear, perturb=perturb)
+        prob = PDEVariationalProblem(W.spaces, W.density, W.batches, W.bc, is_fwd_linear=not nonlinear)
+        for ij in [(ADJOINT, STATE)] + ([(STATE, STATE)] if nonlinear else []):
+            b = prob.block_assembly(ij[0], ij[1], "fp64")
+            assert (b.fused_plan() is not None) is want, (nonlinear, perturb, ij, b.why_two_pass)
+            if not want:
+                assert "nonlinear density" in b.why_two_pass
