@@ -52,6 +52,7 @@ constexpr int FU_SLOTS = 2;     // planes of staged tables
 constexpr int FU_MAXZ = 128;    // layers of a chunk
 constexpr int FU_MAXS2D = 2048;  // stencil places of a tile's lattice plane a unit of height
 constexpr int FU_EB = 8;         // plane entries a thread writes out in one batch
+constexpr int FU_MAXPL = 16;     // pairs of a third-direction type (a block has at most 16)
 
 // The box, the tile grid and the plan arrays (kernel argument).
 struct FusedGeom {
@@ -163,6 +164,11 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
   // lane's choice between them costs a reload of each from the spill lanes.
   __shared__ uint64_t PTAB[7];
   __shared__ int32_t EOFF[FU_MAXM];  // every element slot's element within its layer, ex + n_x ey
+  // (general form) the pair lists of FusedGeom, read per lane: an element's list depends on its
+  // alignment, and a per-lane read of the kernel arguments is a global load and a wait for every
+  // load in flight
+  __shared__ int32_t PLS[2 * 4 * FU_MAXPL];
+  __shared__ int32_t PNS[2 * 4];
 
   const int nu = P.nd_u, nm = P.nd_m;
   const bool fields = SP.need_fields != 0;
@@ -241,6 +247,11 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
       PTAB[6] = (uint64_t)P.edofs_u;
     }
     for (int m = tid; m < M; m += BS) EOFF[m] = (xs + m % cx) + G.nx * (ys + m / cx);
+    if constexpr (!AL) {
+      for (int i = tid; i < 2 * 4 * FU_MAXPL; i += BS)
+        PLS[i] = G.pl[i / (4 * FU_MAXPL)][(i / FU_MAXPL) % 4][i % FU_MAXPL];
+      for (int i = tid; i < 2 * 4; i += BS) PNS[i] = G.pn[i / 4][i % 4];
+    }
     if (fields) {
       for (int i = tid; i < 2 * P.n1_u * NQ; i += BS) T1U[i] = P.tab1_u[i];
       for (int i = tid; i < nu; i += BS) UT[P.lex_u[i]] = i;
@@ -566,7 +577,7 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
       const CT* Wl = Wreg(m);
 #pragma unroll 1
       for (int t = 0; t < 4; ++t) {
-        const int np = G.pn[kind][t];
+        const int np = AL ? G.pn[1][t] : PNS[kind * 4 + t];
         if (np == 0) continue;
         CT s2[NC][NQ];
 #pragma unroll
@@ -574,7 +585,7 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
 #pragma unroll
           for (int q = 0; q < NQ; ++q) s2[i][q] = 0;
         for (int k = 0; k < np; ++k) {
-          const int pw = G.pl[kind][t][k];
+          const int pw = AL ? G.pl[1][t][k] : PLS[(kind * 4 + t) * FU_MAXPL + k];
           const int d1 = pw & 3, d2 = (pw >> 2) & 3;
           const CT* wv = Wl + (pw >> 4) * NQ3;
           const CT* f1 = PT + ((d1 * NT + a1) * NC + b1) * NQ;
@@ -796,7 +807,7 @@ static FusedSizes fu_pick(const hm_problem& P, const SumfactPlan& SP, int nslot,
 // the static LDS of k_fused (task table, row places, widths, flags), left out of its dynamic LDS
 // (with 256 bytes for the alignment of the arrays)
 constexpr size_t FU_STATIC_LDS =
-    sizeof(int32_t) * (32 + FU_MAXT + FU_MAXL * FU_MAXL + 2 * FU_MAXL + 2 * FU_MAXM + 4) + FU_MAXS2D +
+    sizeof(int32_t) * (32 + FU_MAXT + FU_MAXL * FU_MAXL + 2 * FU_MAXL + 2 * FU_MAXM + 4 + 8 * FU_MAXPL + 8) + FU_MAXS2D +
     7 * sizeof(uint64_t) + 256;
 
 // The dynamic LDS a block may take when `blocks` blocks share a compute unit.
@@ -947,9 +958,11 @@ static int fu_run(const hm_problem& P, int i, int j, const SumfactPlan& SP, cons
         if (!((SP.tmask[t] >> p) & 1) || (kind == 1 && ((SP.offdiag >> p) & 1))) continue;
         const int d1 = ((SP.pr[p] == 1) << 1) | (SP.pc[p] == 1);
         const int d2 = ((SP.pr[p] == 2) << 1) | (SP.pc[p] == 2);
-        G.pl[kind][t][n++] = d1 | (d2 << 2) | (W.ws[p] << 4);
+        if (n < FU_MAXPL) G.pl[kind][t][n] = d1 | (d2 << 2) | (W.ws[p] << 4);
+        ++n;
       }
       G.pn[kind][t] = n;
+      if (n > FU_MAXPL) HM_FAIL(HM_ERR_UNSUPPORTED, "more than %d pairs of one third-direction type", FU_MAXPL);
     }
   // layers a chunk: the fewest rounds of blocks over the compute units (NB blocks on each)
   // times the layers a block sweeps (one more for the halo layer of a cut column)
