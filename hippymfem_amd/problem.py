@@ -226,7 +226,7 @@ class BlockAssembly:
                     not self._fp32_two_blocks():
                 # single precision: FP32 element matrices halve the two-pass route's traffic;
                 # the fused kernel gains on it only where two of its blocks share a compute
-                # unit (64^3 P2 A: box 0.56 ms fused against 0.83 ms two-pass, perturbed 0.85
+                # unit (64^3 P2 A: box 0.55 ms fused against 0.83 ms two-pass, perturbed 0.81
                 # against 1.05; with one block, perturbed: 1.10 ms fused)
                 why = "single precision with one fused block a compute unit (the two-pass " \
                       "route is faster)"
