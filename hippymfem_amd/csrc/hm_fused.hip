@@ -575,10 +575,11 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
       // element's leave out the pairs of two different reference derivatives)
       const int kind = (AL || EFLAG[m]) ? 1 : 0;
       const CT* Wl = Wreg(m);
-#pragma unroll 1
-      for (int t = 0; t < 4; ++t) {
+      // one third-direction type t (FP64 aligned form: the four unrolled, which lets the
+      // compiler read every list length ahead; elsewhere that costs more than it gains)
+      auto type_step = [&](int t) {
         const int np = AL ? G.pn[1][t] : PNS[kind * 4 + t];
-        if (np == 0) continue;
+        if (np == 0) return;
         CT s2[NC][NQ];
 #pragma unroll
         for (int i = 0; i < NC; ++i)
@@ -669,6 +670,13 @@ k_fused(const hm_problem P, int s, int j, SumfactPlan SP, FusedGeom G, FusedSize
             }
           }
         }
+      };
+      if constexpr (AL && sizeof(CT) == 8) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) type_step(t);
+      } else {
+#pragma unroll 1
+        for (int t = 0; t < 4; ++t) type_step(t);
       }
     }
     rederive();
